@@ -1,0 +1,150 @@
+/*
+ * lq_amd.h -- C-ABI of the MI355X state-vector engine for the lightning.qubit
+ * hot path (gate application, expval/probs, adjoint Jacobian).
+ *
+ * Every entry point returns 0 on success and non-zero on failure; the
+ * failure message (format "[file][Line:n][Method:f]: Error in PennyLane
+ * Lightning: <msg>", as core/utils/Error.hpp:111-140) is available from
+ * lq_last_error() on the calling thread. Plain pointers and sizes only.
+ *
+ * Reference paths are relative to pennylane_lightning/core/
+ * (lq/ = simulators/lightning_qubit/). Wire w is amplitude-index bit n-1-w
+ * (lq/StateVectorLQubit.hpp:950). Matrices are row-major with wires[0] the
+ * most significant bit of the local index; they are complex64 for LQ_C64
+ * states and complex128 for LQ_C128 states, like the reference bindings.
+ *
+ * Gate application is queued and executed in fused LDS passes when a result
+ * is needed (a measurement, lq_sv_get_data, lq_sv_synchronize). Argument
+ * errors are reported by the call that queues the gate.
+ *
+ * Handles are not thread-safe (the reference's state vectors are not
+ * either, see lq/StateVectorLQubit.hpp); use one handle per thread.
+ */
+#ifndef LQ_AMD_H
+#define LQ_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lq_sv_s *lq_sv_t;   /* device state vector + stream + op queue */
+typedef struct lq_obs_s *lq_obs_t; /* observable tree */
+typedef struct lq_ops_s *lq_ops_t; /* operation list (OpsData) */
+
+enum { LQ_C64 = 0, LQ_C128 = 1 };
+
+/* Message of the last failure on this thread. */
+const char *lq_last_error(void);
+
+/* Engine identification, replaces backend_info()/runtime_info()
+ * (core/bindings/Bindings.hpp:284-367, lq/bindings/LQubitBindings.hpp:464-475). */
+const char *lq_backend_info(void);
+int lq_device_count(int *out);
+
+/* ---- State vector --------------------------------------------------------
+ * Replaces StateVectorLQubitManaged<T>(num_qubits)
+ * (lq/StateVectorLQubitManaged.hpp:75-82; binding core/bindings/Bindings.hpp:876). */
+int lq_sv_create(uint32_t num_qubits, int dtype, lq_sv_t *out);
+int lq_sv_destroy(lq_sv_t sv);
+int lq_sv_num_qubits(lq_sv_t sv, uint32_t *out);
+int lq_sv_dtype(lq_sv_t sv, int *out);
+/* resetStateVector (lq/StateVectorLQubit.hpp:959-963) */
+int lq_sv_reset(lq_sv_t sv);
+/* setBasisState(state, wires) (lq/StateVectorLQubit.hpp:943-953) */
+int lq_sv_set_basis_state(lq_sv_t sv, const uint32_t *bits, const uint32_t *wires, uint32_t nw);
+/* setStateVector(state, wires): writes the 2^nw amplitudes where the other
+ * wires are 0, leaves the rest (lq/StateVectorLQubit.hpp:1005-1036) */
+int lq_sv_set_state_vector(lq_sv_t sv, const void *state, const uint32_t *wires, uint32_t nw);
+/* updateData / getState copy in and out (lq/bindings/LQubitBindings.hpp:196-243) */
+int lq_sv_update_data(lq_sv_t sv, const void *host, uint64_t len);
+int lq_sv_get_data(lq_sv_t sv, void *host, uint64_t len);
+/* copy of another state of the same size and precision (device to device) */
+int lq_sv_copy(lq_sv_t dst, lq_sv_t src);
+/* executes queued gates and waits for the device */
+int lq_sv_synchronize(lq_sv_t sv);
+/* device address of the amplitudes (executes queued gates first) */
+int lq_sv_device_ptr(lq_sv_t sv, void **out);
+
+/* applyOperation(name, [controlled_wires, controlled_values,] wires, inverse, params)
+ * (lq/StateVectorLQubit.hpp:378-423; binding core/bindings/Bindings.hpp:213-277).
+ * nc = 0 for the plain form. "Identity" is a no-op. */
+int lq_sv_apply_op(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
+                   const uint8_t *ctrl_vals, uint32_t nc, int inverse, const double *params, uint32_t np);
+/* applyMatrix / applyControlledMatrix (lq/StateVectorLQubit.hpp:571-710;
+ * binding core/bindings/Bindings.hpp:152-200); at most 10 target wires */
+int lq_sv_apply_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
+                       const uint8_t *ctrl_vals, uint32_t nc, int inverse);
+/* applyPauliRot(wires, inverse, {theta}, word) (lq/StateVectorLQubit.hpp:492-501) */
+int lq_sv_apply_pauli_rot(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int inverse, double theta,
+                          const char *word);
+/* applyGenerator (lq/StateVectorLQubit.hpp:527-560): state <- G state, the
+ * non-control subspace zeroed; *scale receives s with U = exp(i s theta G) */
+int lq_sv_apply_generator(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
+                          const uint8_t *ctrl_vals, uint32_t nc, int adj, double *scale);
+/* normalize / collapse (lq/StateVectorLQubit.hpp:881-921) */
+int lq_sv_normalize(lq_sv_t sv);
+int lq_sv_collapse(lq_sv_t sv, uint32_t wire, int branch);
+
+/* ---- Measurements (lq/measurements/MeasurementsLQubit.hpp) ---------------- */
+/* probs() when all != 0, else probs(wires) (:90-163); out has 2^n or 2^nw doubles */
+int lq_probs(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int all, double *out);
+/* expval(name, wires) for Identity/PauliX/PauliY/PauliZ/Hadamard (:247-305) */
+int lq_expval_named(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, double *out);
+/* several one-wire named expvals in one read of the state */
+int lq_expval_named_batch(lq_sv_t sv, uint32_t count, const char *const *names, const uint32_t *wires,
+                          double *out);
+/* expval(matrix, wires) (:213-238) */
+int lq_expval_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, double *out);
+/* expval(Observable) (:372-394) */
+int lq_expval_obs(lq_sv_t sv, lq_obs_t obs, double *out);
+
+/* ---- Observables (core/observables/Observables.hpp:128-584,
+ *      lq/observables/ObservablesLQubit.hpp:52-366) ---------------------- */
+int lq_obs_named(const char *name, const uint32_t *wires, uint32_t nw, lq_obs_t *out);
+/* matrix is complex128, row-major 2^nw x 2^nw */
+int lq_obs_hermitian(const void *matrix_c128, const uint32_t *wires, uint32_t nw, lq_obs_t *out);
+int lq_obs_tensor(const lq_obs_t *factors, uint32_t count, lq_obs_t *out);
+int lq_obs_hamiltonian(const double *coeffs, const lq_obs_t *terms, uint32_t count, lq_obs_t *out);
+int lq_obs_destroy(lq_obs_t obs);
+
+/* ---- Operation list: OpsData (core/algorithms/JacobianData.hpp:39-253),
+ * built like create_ops_list (core/bindings/Bindings.hpp:772-799). Arrays
+ * are flattened with offsets of length nops+1; matrices are interleaved
+ * complex128 (empty for named gates). */
+int lq_ops_create(uint32_t nops, const char *const *names, const uint32_t *param_off, const double *params,
+                  const uint32_t *wire_off, const uint32_t *wires, const uint8_t *inverses, const uint32_t *mat_off,
+                  const double *matrices, const uint32_t *ctrl_off, const uint32_t *ctrls, const uint8_t *ctrl_vals,
+                  lq_ops_t *out);
+int lq_ops_destroy(lq_ops_t ops);
+int lq_ops_num_par_ops(lq_ops_t ops, uint64_t *out);
+/* apply every op of the list to the state (AdjointJacobianBase::applyOperations,
+ * core/algorithms/AdjointJacobianBase.hpp:75-100) */
+int lq_sv_apply_ops(lq_sv_t sv, lq_ops_t ops, int adjoint);
+
+/* ---- Adjoint Jacobian: AdjointJacobian<SV>::adjointJacobian
+ * (lq/algorithms/AdjointJacobianLQubit.hpp:347-491; binding
+ * core/bindings/Bindings.hpp:698-729). `sv` holds the final state (or the
+ * initial state with apply_operations != 0); trainable indexes the ops with
+ * parameters. jac receives nobs*ntp doubles in [obs][param] order; expvals,
+ * if not NULL, receives <obs> on the final state. sv is not modified. */
+int lq_adjoint_jacobian(lq_sv_t sv, lq_ops_t ops, const lq_obs_t *obs, uint32_t nobs, const uint64_t *trainable,
+                        uint32_t ntp, int apply_operations, double *jac, double *expvals);
+/* ---- Instrumentation (no reference counterpart) ----------------------------
+ * When enabled, every tile-pass kernel launch on the handle's stream is
+ * bracketed by HIP events; profile_read totals the measured kernel time, the
+ * number of passes and their algorithmic HBM bytes (amplitudes read +
+ * written). lq_sv_stream gives the handle's HIP stream (hipStream_t). */
+int lq_sv_profile(lq_sv_t sv, int enable);
+int lq_sv_profile_read(lq_sv_t sv, int reset, uint64_t *passes, double *ms, double *bytes, double *max_ms);
+int lq_sv_stream(lq_sv_t sv, void **out);
+
+/* free the adjoint's work states held by the handle */
+int lq_sv_release_workspace(lq_sv_t sv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LQ_AMD_H */

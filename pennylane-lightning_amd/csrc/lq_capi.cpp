@@ -1,0 +1,320 @@
+// extern "C" boundary (include/lq_amd.h) over the engine.
+#include "lq_amd.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "lq_engine.hpp"
+
+using namespace lqamd;
+
+struct lq_sv_s {
+    std::unique_ptr<StateVector> sv;
+};
+struct lq_obs_s {
+    std::shared_ptr<Obs> obs;
+};
+struct lq_ops_s {
+    OpsList list;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F> int guard(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return 1;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return 1;
+    }
+}
+
+StateVector &S(lq_sv_t h) {
+    LQ_ABORT_IF(h == nullptr || !h->sv, "null state vector handle");
+    return *h->sv;
+}
+} // namespace
+
+extern "C" {
+
+const char *lq_last_error(void) { return g_last_error.c_str(); }
+
+const char *lq_backend_info(void) { return "lightning.qubit hot path on MI355X (gfx950, HIP tile programs)"; }
+
+int lq_device_count(int *out) {
+    return guard([&] { hip_check(hipGetDeviceCount(out), "hipGetDeviceCount"); });
+}
+
+int lq_sv_create(uint32_t num_qubits, int dtype, lq_sv_t *out) {
+    return guard([&] {
+        auto h = std::make_unique<lq_sv_s>();
+        h->sv = std::make_unique<StateVector>(static_cast<int>(num_qubits), dtype);
+        *out = h.release();
+    });
+}
+
+int lq_sv_destroy(lq_sv_t sv) {
+    return guard([&] { delete sv; });
+}
+
+int lq_sv_num_qubits(lq_sv_t sv, uint32_t *out) {
+    return guard([&] { *out = static_cast<uint32_t>(S(sv).n()); });
+}
+
+int lq_sv_dtype(lq_sv_t sv, int *out) {
+    return guard([&] { *out = S(sv).dtype(); });
+}
+
+int lq_sv_reset(lq_sv_t sv) {
+    return guard([&] { S(sv).reset(); });
+}
+
+int lq_sv_set_basis_state(lq_sv_t sv, const uint32_t *bits, const uint32_t *wires, uint32_t nw) {
+    return guard([&] { S(sv).set_basis_state(bits, wires, static_cast<int>(nw)); });
+}
+
+int lq_sv_set_state_vector(lq_sv_t sv, const void *state, const uint32_t *wires, uint32_t nw) {
+    return guard([&] { S(sv).set_state_vector(state, wires, static_cast<int>(nw)); });
+}
+
+int lq_sv_update_data(lq_sv_t sv, const void *host, uint64_t len) {
+    return guard([&] { S(sv).update_data(host, len); });
+}
+
+int lq_sv_get_data(lq_sv_t sv, void *host, uint64_t len) {
+    return guard([&] { S(sv).get_data(host, len); });
+}
+
+int lq_sv_copy(lq_sv_t dst, lq_sv_t src) {
+    return guard([&] { S(dst).copy_from(S(src)); });
+}
+
+int lq_sv_synchronize(lq_sv_t sv) {
+    return guard([&] { S(sv).synchronize(); });
+}
+
+int lq_sv_device_ptr(lq_sv_t sv, void **out) {
+    return guard([&] {
+        S(sv).synchronize();
+        *out = S(sv).data();
+    });
+}
+
+int lq_sv_apply_op(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
+                   const uint8_t *ctrl_vals, uint32_t nc, int inverse, const double *params, uint32_t np) {
+    return guard([&] {
+        S(sv).apply_named(name, wires, static_cast<int>(nw), ctrls, ctrl_vals, static_cast<int>(nc), inverse != 0,
+                          params, static_cast<int>(np));
+    });
+}
+
+int lq_sv_apply_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
+                       const uint8_t *ctrl_vals, uint32_t nc, int inverse) {
+    return guard([&] {
+        S(sv).apply_matrix(matrix, wires, static_cast<int>(nw), ctrls, ctrl_vals, static_cast<int>(nc), inverse != 0);
+    });
+}
+
+int lq_sv_apply_pauli_rot(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int inverse, double theta,
+                          const char *word) {
+    return guard([&] { S(sv).apply_pauli_rot(wires, static_cast<int>(nw), inverse != 0, theta, word); });
+}
+
+int lq_sv_apply_generator(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
+                          const uint8_t *ctrl_vals, uint32_t nc, int adj, double *scale) {
+    (void)adj; // the generator does not depend on adj (GateImplementationsLM.hpp:2181-2952)
+    return guard([&] {
+        *scale = S(sv).apply_generator(name, wires, static_cast<int>(nw), ctrls, ctrl_vals, static_cast<int>(nc));
+    });
+}
+
+int lq_sv_normalize(lq_sv_t sv) {
+    return guard([&] { S(sv).normalize(); });
+}
+
+int lq_sv_collapse(lq_sv_t sv, uint32_t wire, int branch) {
+    return guard([&] { S(sv).collapse(wire, branch != 0); });
+}
+
+int lq_probs(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int all, double *out) {
+    return guard([&] {
+        const auto p = S(sv).probs(wires, static_cast<int>(nw), all != 0);
+        std::memcpy(out, p.data(), p.size() * sizeof(double));
+    });
+}
+
+int lq_expval_named(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, double *out) {
+    return guard([&] { *out = S(sv).expval_named(name, wires, static_cast<int>(nw)); });
+}
+
+int lq_expval_named_batch(lq_sv_t sv, uint32_t count, const char *const *names, const uint32_t *wires,
+                          double *out) {
+    return guard([&] {
+        std::vector<std::string> nm(names, names + count);
+        std::vector<uint32_t> w(wires, wires + count);
+        const auto v = S(sv).expval_named_batch(nm, w);
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    });
+}
+
+int lq_expval_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, double *out) {
+    return guard([&] { *out = S(sv).expval_matrix(matrix, wires, static_cast<int>(nw)); });
+}
+
+int lq_expval_obs(lq_sv_t sv, lq_obs_t obs, double *out) {
+    return guard([&] {
+        LQ_ABORT_IF(!obs, "null observable");
+        *out = S(sv).expval_obs(*obs->obs);
+    });
+}
+
+int lq_obs_named(const char *name, const uint32_t *wires, uint32_t nw, lq_obs_t *out) {
+    return guard([&] {
+        GateId g;
+        const std::string nm(name);
+        LQ_ABORT_IF(!gate_from_name(nm, &g), (std::string("Operation does not exist for ") + nm).c_str());
+        LQ_ABORT_IF(gate_info(g).nwires > 0 && static_cast<int>(nw) != gate_info(g).nwires,
+                    "wrong number of wires for the named observable");
+        auto o = std::make_shared<Obs>();
+        o->kind = Obs::Named;
+        o->name = nm;
+        o->wires.assign(wires, wires + nw);
+        *out = new lq_obs_s{o};
+    });
+}
+
+int lq_obs_hermitian(const void *matrix_c128, const uint32_t *wires, uint32_t nw, lq_obs_t *out) {
+    return guard([&] {
+        LQ_ABORT_IF(nw < 1 || nw > 10, "Hermitian observables are supported on 1 to 10 wires");
+        auto o = std::make_shared<Obs>();
+        o->kind = Obs::Hermitian;
+        o->wires.assign(wires, wires + nw);
+        const size_t d = static_cast<size_t>(1) << nw;
+        const auto *m = static_cast<const cd *>(matrix_c128);
+        o->matrix.assign(m, m + d * d);
+        *out = new lq_obs_s{o};
+    });
+}
+
+int lq_obs_tensor(const lq_obs_t *factors, uint32_t count, lq_obs_t *out) {
+    return guard([&] {
+        auto o = std::make_shared<Obs>();
+        o->kind = Obs::Tensor;
+        std::vector<uint32_t> seen;
+        for (uint32_t i = 0; i < count; ++i) {
+            LQ_ABORT_IF(!factors[i], "null observable");
+            for (auto w : factors[i]->obs->all_wires()) {
+                for (auto s : seen) LQ_ABORT_IF(s == w, "All wires in observables must be disjoint.");
+                seen.push_back(w);
+            }
+            o->terms.push_back(factors[i]->obs);
+        }
+        LQ_ABORT_IF(count == 1 && factors[0]->obs->kind == Obs::Tensor,
+                    "A new TensorProdObsBase observable cannot be created from a single TensorProdObsBase.");
+        *out = new lq_obs_s{o};
+    });
+}
+
+int lq_obs_hamiltonian(const double *coeffs, const lq_obs_t *terms, uint32_t count, lq_obs_t *out) {
+    return guard([&] {
+        auto o = std::make_shared<Obs>();
+        o->kind = Obs::Ham;
+        for (uint32_t i = 0; i < count; ++i) {
+            LQ_ABORT_IF(!terms[i], "null observable");
+            o->terms.push_back(terms[i]->obs);
+            o->coeffs.push_back(coeffs[i]);
+        }
+        *out = new lq_obs_s{o};
+    });
+}
+
+int lq_obs_destroy(lq_obs_t obs) {
+    return guard([&] { delete obs; });
+}
+
+int lq_ops_create(uint32_t nops, const char *const *names, const uint32_t *param_off, const double *params,
+                  const uint32_t *wire_off, const uint32_t *wires, const uint8_t *inverses, const uint32_t *mat_off,
+                  const double *matrices, const uint32_t *ctrl_off, const uint32_t *ctrls, const uint8_t *ctrl_vals,
+                  lq_ops_t *out) {
+    return guard([&] {
+        auto h = std::make_unique<lq_ops_s>();
+        for (uint32_t i = 0; i < nops; ++i) {
+            OpRec r;
+            r.name = names[i];
+            r.named = gate_from_name(r.name, &r.gid);
+            r.params.assign(params + param_off[i], params + param_off[i + 1]);
+            r.wires.assign(wires + wire_off[i], wires + wire_off[i + 1]);
+            r.inverse = inverses[i] != 0;
+            for (uint32_t j = mat_off[i]; j < mat_off[i + 1]; ++j) r.matrix.emplace_back(matrices[2 * j], matrices[2 * j + 1]);
+            r.ctrls.assign(ctrls + ctrl_off[i], ctrls + ctrl_off[i + 1]);
+            r.cvals.assign(ctrl_vals + ctrl_off[i], ctrl_vals + ctrl_off[i + 1]);
+            if (!r.params.empty()) ++h->list.num_par_ops;
+            h->list.ops.push_back(std::move(r));
+        }
+        *out = h.release();
+    });
+}
+
+int lq_ops_destroy(lq_ops_t ops) {
+    return guard([&] { delete ops; });
+}
+
+int lq_ops_num_par_ops(lq_ops_t ops, uint64_t *out) {
+    return guard([&] { *out = ops->list.num_par_ops; });
+}
+
+int lq_sv_apply_ops(lq_sv_t sv, lq_ops_t ops, int adjoint) {
+    return guard([&] {
+        LQ_ABORT_IF(!ops, "null ops");
+        if (adjoint) {
+            for (auto it = ops->list.ops.rbegin(); it != ops->list.ops.rend(); ++it) S(sv).apply_oprec(*it, true);
+        } else {
+            for (const auto &op : ops->list.ops) S(sv).apply_oprec(op, false);
+        }
+    });
+}
+
+int lq_adjoint_jacobian(lq_sv_t sv, lq_ops_t ops, const lq_obs_t *obs, uint32_t nobs, const uint64_t *trainable,
+                        uint32_t ntp, int apply_operations, double *jac, double *expvals) {
+    return guard([&] {
+        LQ_ABORT_IF(!ops, "null ops");
+        std::vector<std::shared_ptr<Obs>> o;
+        for (uint32_t i = 0; i < nobs; ++i) {
+            LQ_ABORT_IF(!obs[i], "null observable");
+            o.push_back(obs[i]->obs);
+        }
+        std::vector<uint64_t> tp(trainable, trainable + ntp);
+        adjoint_jacobian(S(sv), ops->list, o, tp, apply_operations != 0, jac, expvals);
+    });
+}
+
+int lq_sv_profile(lq_sv_t sv, int enable) {
+    return guard([&] { S(sv).profile(enable != 0); });
+}
+
+int lq_sv_profile_read(lq_sv_t sv, int reset, uint64_t *passes, double *ms, double *bytes, double *max_ms) {
+    return guard([&] {
+        const auto st = S(sv).profile_read(reset != 0);
+        if (passes) *passes = st.passes;
+        if (ms) *ms = st.ms;
+        if (bytes) *bytes = st.bytes;
+        if (max_ms) *max_ms = st.max_ms;
+    });
+}
+
+int lq_sv_stream(lq_sv_t sv, void **out) {
+    return guard([&] { *out = static_cast<void *>(S(sv).stream()); });
+}
+
+int lq_sv_release_workspace(lq_sv_t sv) {
+    return guard([&] { S(sv).release_workspace(); });
+}
+
+} // extern "C"

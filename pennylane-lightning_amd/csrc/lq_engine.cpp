@@ -1,0 +1,956 @@
+// Device state vector and executor (host side).
+#include "lq_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <set>
+
+namespace lqamd {
+
+void abort_with(const char *msg, const char *file, int line, const char *func) {
+    std::string s = std::string("[") + file + "][Line:" + std::to_string(line) + "][Method:" + func +
+                    "]: Error in PennyLane Lightning: " + msg;
+    throw LightningException(s);
+}
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        const std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        LQ_ABORT(m.c_str());
+    }
+}
+
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+void DevBuf::ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    hip_check(hipMalloc(&p, b), "hipMalloc");
+    bytes = b;
+}
+
+HostPinned::~HostPinned() {
+    if (p) (void)hipHostFree(p);
+}
+void HostPinned::ensure(size_t b) {
+    if (b <= bytes) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    hip_check(hipHostMalloc(&p, b, hipHostMallocDefault), "hipHostMalloc");
+    bytes = b;
+}
+
+EngineTuning &tuning() {
+    static EngineTuning t = [] {
+        EngineTuning e;
+        if (const char *s = std::getenv("LQ_TILE_BITS")) e.kt_max = std::atoi(s);
+        if (const char *s = std::getenv("LQ_LOW_BITS")) e.low_bits = std::atoi(s);
+        if (const char *s = std::getenv("LQ_PASS_BUDGET")) e.budget = std::atof(s);
+        if (const char *s = std::getenv("LQ_LDS_BYTES")) e.lds_bytes = static_cast<size_t>(std::atol(s));
+        return e;
+    }();
+    return t;
+}
+
+namespace {
+template <typename F> void dispatch(int dtype, F &&f) {
+    if (dtype) f(double{});
+    else f(float{});
+}
+
+void check_wires(int n, const uint32_t *wires, int nw, const uint32_t *ctrls, int nc) {
+    uint64_t seen = 0;
+    for (int i = 0; i < nw + nc; ++i) {
+        const uint32_t w = i < nw ? wires[i] : ctrls[i - nw];
+        LQ_ABORT_IF(static_cast<int>(w) >= n, "wire index out of range");
+        LQ_ABORT_IF((seen >> w) & 1ull, "`controlled_wires` and `target wires` must be disjoint.");
+        seen |= 1ull << w;
+    }
+}
+
+const std::vector<cd> &named_obs_matrix(const std::string &name) {
+    static const std::vector<cd> X{0, 1, 1, 0}, Y{0, cd(0, -1), cd(0, 1), 0}, Z{1, 0, 0, -1},
+        H{M_SQRT1_2, M_SQRT1_2, M_SQRT1_2, -M_SQRT1_2}, I{1, 0, 0, 1};
+    if (name == "PauliX") return X;
+    if (name == "PauliY") return Y;
+    if (name == "PauliZ") return Z;
+    if (name == "Hadamard") return H;
+    if (name == "Identity") return I;
+    LQ_ABORT((std::string("Expval does not exist for named observable ") + name).c_str());
+}
+
+LOp make_dot(LOp op, uint32_t sa, uint32_t sb, int slot) {
+    op.flags = (op.flags & ~TF_ZERO_UNCTRL) | TF_DOT;
+    op.sa = sa;
+    op.sb = sb;
+    op.slot = slot;
+    return op;
+}
+
+LOp identity_dot(uint32_t sa, uint32_t sb, int slot) {
+    LOp op;
+    op.kind = TOP_DIAG;
+    op.k = 0;
+    op.coef = {1.0};
+    return make_dot(op, sa, sb, slot);
+}
+
+LOp pauli_op(const PauliWord &w) {
+    LOp op;
+    op.kind = TOP_PAULI;
+    op.xmask = w.x;
+    op.yzmask = w.yz;
+    op.ny = w.ny;
+    cd ph = 1.0;
+    for (int i = 0; i < (w.ny & 3); ++i) ph *= cd(0, -1);
+    op.coef = {ph};
+    return op;
+}
+} // namespace
+
+std::vector<uint32_t> Obs::all_wires() const {
+    std::set<uint32_t> s;
+    if (kind == Named || kind == Hermitian) s.insert(wires.begin(), wires.end());
+    for (const auto &t : terms)
+        for (auto w : t->all_wires()) s.insert(w);
+    return {s.begin(), s.end()};
+}
+
+bool obs_to_pauli(const Obs &o, int n, std::vector<PauliWord> *out) {
+    auto check = [n](uint32_t w) { LQ_ABORT_IF(static_cast<int>(w) >= n, "observable wire out of range"); };
+    switch (o.kind) {
+    case Obs::Named: {
+        check(o.wires.at(0));
+        const uint64_t b = 1ull << phys_bit(n, o.wires[0]);
+        if (o.name == "Identity") out->push_back({1.0, 0, 0, 0});
+        else if (o.name == "PauliX") out->push_back({1.0, b, 0, 0});
+        else if (o.name == "PauliY") out->push_back({1.0, b, b, 1});
+        else if (o.name == "PauliZ") out->push_back({1.0, 0, b, 0});
+        else if (o.name == "Hadamard") {
+            out->push_back({M_SQRT1_2, b, 0, 0});
+            out->push_back({M_SQRT1_2, 0, b, 0});
+        } else return false;
+        return true;
+    }
+    case Obs::Hermitian: return false;
+    case Obs::Tensor: {
+        std::vector<PauliWord> acc{{1.0, 0, 0, 0}};
+        for (const auto &f : o.terms) {
+            std::vector<PauliWord> fw;
+            if (!obs_to_pauli(*f, n, &fw)) return false;
+            std::vector<PauliWord> next;
+            for (const auto &a : acc)
+                for (const auto &b : fw) {
+                    LQ_ABORT_IF((a.x | a.yz) & (b.x | b.yz), "All wires in observables must be disjoint.");
+                    next.push_back({a.coeff * b.coeff, a.x | b.x, a.yz | b.yz, a.ny + b.ny});
+                }
+            acc.swap(next);
+        }
+        out->insert(out->end(), acc.begin(), acc.end());
+        return true;
+    }
+    case Obs::Ham: {
+        std::vector<PauliWord> all;
+        for (size_t i = 0; i < o.terms.size(); ++i) {
+            std::vector<PauliWord> tw;
+            if (!obs_to_pauli(*o.terms[i], n, &tw)) return false;
+            for (auto &w : tw) w.coeff *= o.coeffs[i];
+            all.insert(all.end(), tw.begin(), tw.end());
+        }
+        out->insert(out->end(), all.begin(), all.end());
+        return true;
+    }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+StateVector::StateVector(int n, int dtype) : n_(n), dtype_(dtype ? 1 : 0) {
+    LQ_ABORT_IF(n < 1 || n > 40, "number of qubits must be in [1, 40]");
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&staged_, hipEventDisableTiming), "hipEventCreate");
+    hipError_t e = hipMalloc(&d_, bytes());
+    if (e != hipSuccess) {
+        d_ = nullptr;
+        (void)hipEventDestroy(staged_);
+        (void)hipStreamDestroy(stream_);
+        hip_check(e, "hipMalloc(state vector)");
+    }
+    reset();
+}
+
+StateVector::~StateVector() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto &p : pending_) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ev_free_) (void)hipEventDestroy(e);
+    ws_.clear();
+    if (d_) (void)hipFree(d_);
+    if (staged_) (void)hipEventDestroy(staged_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+hipEvent_t StateVector::take_event() {
+    if (!ev_free_.empty()) {
+        hipEvent_t e = ev_free_.back();
+        ev_free_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+void StateVector::profile(bool on) {
+    synchronize();
+    prof_ = on;
+}
+
+StateVector::PassStats StateVector::profile_read(bool reset) {
+    synchronize();
+    for (auto &p : pending_) {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+        stats_.passes += 1;
+        stats_.ms += ms;
+        stats_.bytes += p.bytes;
+        stats_.max_ms = std::max(stats_.max_ms, static_cast<double>(ms));
+        ev_free_.push_back(p.a);
+        ev_free_.push_back(p.b);
+    }
+    pending_.clear();
+    PassStats out = stats_;
+    if (reset) stats_ = PassStats{};
+    return out;
+}
+
+void *StateVector::workspace(int i) {
+    while (static_cast<int>(ws_.size()) <= i) ws_.push_back(std::make_unique<DevBuf>());
+    ws_[i]->ensure(bytes());
+    return ws_[i]->p;
+}
+
+void StateVector::release_workspace() {
+    synchronize();
+    ws_.clear();
+}
+
+namespace {
+// Dynamic LDS the tile kernel may request: the tuning budget, within the
+// device's per-workgroup limit less the kernel's static reduction buffer.
+size_t lds_budget() {
+    static const size_t v = [] {
+        int dev = 0, lim = 0;
+        size_t b = tuning().lds_bytes;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lim > 0)
+            b = std::min(b, static_cast<size_t>(lim) - 1024);
+        return b;
+    }();
+    return v;
+}
+} // namespace
+
+int StateVector::kt_for(int ns) const {
+    const EngineTuning &t = tuning();
+    const size_t budget = lds_budget();
+    int kt = 0;
+    while ((elem() * static_cast<size_t>(ns) << (kt + 1)) <= budget) ++kt;
+    if (t.kt_max > 0) kt = std::min(kt, t.kt_max);
+    return std::min(kt, 13);
+}
+
+void StateVector::reset() { set_basis_index(0); }
+
+void StateVector::set_basis_index(uint64_t index) {
+    LQ_ABORT_IF(index > length() - 1, "Invalid index");
+    queue_.clear();
+    dispatch(dtype_, [&](auto t) { launch_fill_basis<decltype(t)>(d_, n_, index, stream_); });
+}
+
+void StateVector::set_basis_state(const uint32_t *bits, const uint32_t *wires, int nw) {
+    uint64_t index = 0;
+    for (int k = 0; k < nw; ++k) {
+        LQ_ABORT_IF(static_cast<int>(wires[k]) >= n_, "wire index out of range");
+        index |= static_cast<uint64_t>(bits[k] != 0) << (n_ - 1 - static_cast<int>(wires[k]));
+    }
+    set_basis_index(index);
+}
+
+void StateVector::set_state_vector(const void *host, const uint32_t *wires, int nw) {
+    check_wires(n_, wires, nw, nullptr, 0);
+    flush();
+    const size_t b = elem() << nw;
+    small_.ensure(b + 256 * sizeof(int));
+    hip_check(hipMemcpyAsync(small_.p, host, b, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync");
+    std::vector<int> bits(nw);
+    for (int j = 0; j < nw; ++j) bits[j] = phys_bit(n_, wires[j]);
+    dispatch(dtype_, [&](auto t) { launch_set_subspace<decltype(t)>(d_, n_, small_.p, bits.data(), nw, stream_); });
+    synchronize();
+}
+
+void StateVector::update_data(const void *host, uint64_t len) {
+    LQ_ABORT_IF(len != length(), "The size of provided data must be equal to the statevector size.");
+    queue_.clear();
+    hip_check(hipMemcpyAsync(d_, host, bytes(), hipMemcpyHostToDevice, stream_), "hipMemcpyAsync");
+    synchronize();
+}
+
+void StateVector::get_data(void *host, uint64_t len) {
+    LQ_ABORT_IF(len != length(), "The size of provided data must be equal to the statevector size.");
+    flush();
+    hip_check(hipMemcpyAsync(host, d_, bytes(), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    synchronize();
+}
+
+void StateVector::copy_from(StateVector &src) {
+    LQ_ABORT_IF(src.n_ != n_ || src.dtype_ != dtype_, "state vectors differ in size or precision");
+    src.flush();
+    queue_.clear();
+    src.synchronize();
+    hip_check(hipMemcpyAsync(d_, src.d_, bytes(), hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync");
+}
+
+void StateVector::enqueue(LOp op) {
+    if (try_merge_1q(queue_, op)) return;
+    queue_.push_back(std::move(op));
+    if (queue_.size() >= 8192) flush();
+}
+
+void StateVector::flush() {
+    if (queue_.empty()) return;
+    std::vector<LOp> q;
+    q.swap(queue_);
+    run(q, {d_}, 0);
+}
+
+void StateVector::synchronize() {
+    flush();
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector<void *> &states, int nslots) {
+    std::vector<cd> result(static_cast<size_t>(nslots), 0.0);
+    if (prog.empty()) return result;
+    const int ns = static_cast<int>(states.size());
+    LQ_ABORT_IF(ns < 1 || ns > kMaxStates, "too many states in one program");
+    const EngineTuning &t = tuning();
+    PlanConfig cfg;
+    cfg.n = n_;
+    cfg.ns = ns;
+    cfg.kt_max = kt_for(ns);
+    cfg.low_bits = t.low_bits >= 0 ? t.low_bits : (dtype_ ? 4 : 5);
+    cfg.budget = t.budget;
+    const std::vector<PassPlan> passes = plan_passes(prog, cfg);
+
+    auto align = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+    std::vector<size_t> op_off(passes.size()), coef_off(passes.size()), part_off(passes.size());
+    size_t ops_bytes = 0, coef_n = 0, part_n = 0;
+    std::vector<SlotRange> ranges(static_cast<size_t>(nslots), SlotRange{0, 0});
+    std::vector<char> seen(static_cast<size_t>(nslots), 0);
+    for (size_t p = 0; p < passes.size(); ++p) {
+        op_off[p] = ops_bytes;
+        ops_bytes += passes[p].ops.size() * sizeof(TileOp);
+        coef_off[p] = coef_n;
+        coef_n += passes[p].coef.size();
+        part_off[p] = part_n;
+        const uint64_t ntiles = 1ull << (n_ - static_cast<int>(passes[p].tbits.size()));
+        for (size_t s = 0; s < passes[p].slot_ids.size(); ++s) {
+            const int id = passes[p].slot_ids[s];
+            LQ_ABORT_IF(id < 0 || id >= nslots || seen[id], "invalid dot slot");
+            seen[id] = 1;
+            ranges[id] = SlotRange{part_n + s * ntiles, ntiles};
+        }
+        part_n += passes[p].slot_ids.size() * ntiles;
+    }
+    const size_t coef_at = align(ops_bytes);
+    const size_t ranges_at = align(coef_at + coef_n * sizeof(double2));
+    const size_t total = align(ranges_at + ranges.size() * sizeof(SlotRange));
+
+    // the staging buffer may still feed the previous upload
+    hip_check(hipEventSynchronize(staged_), "hipEventSynchronize");
+    staging_.ensure(total);
+    char *hs = static_cast<char *>(staging_.p);
+    for (size_t p = 0; p < passes.size(); ++p) {
+        std::memcpy(hs + op_off[p], passes[p].ops.data(), passes[p].ops.size() * sizeof(TileOp));
+        double2 *c = reinterpret_cast<double2 *>(hs + coef_at) + coef_off[p];
+        for (size_t i = 0; i < passes[p].coef.size(); ++i)
+            c[i] = make_double2(passes[p].coef[i].real(), passes[p].coef[i].imag());
+    }
+    if (!ranges.empty()) std::memcpy(hs + ranges_at, ranges.data(), ranges.size() * sizeof(SlotRange));
+    prog_.ensure(total);
+    hip_check(hipMemcpyAsync(prog_.p, hs, total, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
+    hip_check(hipEventRecord(staged_, stream_), "hipEventRecord");
+    if (part_n) partials_.ensure(part_n * sizeof(double2));
+
+    char *dp = static_cast<char *>(prog_.p);
+    for (size_t p = 0; p < passes.size(); ++p) {
+        const PassPlan &pp = passes[p];
+        TilePassArgs a{};
+        for (int s = 0; s < ns; ++s) a.sv[s] = states[s];
+        a.ops = reinterpret_cast<const TileOp *>(dp + op_off[p]);
+        a.coef = reinterpret_cast<const double2 *>(dp + coef_at) + coef_off[p];
+        a.partials = static_cast<double2 *>(partials_.p) + part_off[p];
+        a.nops = static_cast<uint32_t>(pp.ops.size());
+        a.load_mask = pp.load_mask;
+        a.store_mask = pp.store_mask;
+        a.ns = static_cast<uint32_t>(ns);
+        a.g = make_geom(n_, pp.tbits);
+        a.ntiles = 1ull << a.g.nout;
+        const size_t lds = static_cast<size_t>(ns) * (elem() << a.g.kt);
+        Pending pe{};
+        if (prof_) {
+            pe.a = take_event();
+            pe.b = take_event();
+            pe.bytes = static_cast<double>(bytes()) *
+                       (__builtin_popcount(pp.load_mask) + __builtin_popcount(pp.store_mask));
+            hip_check(hipEventRecord(pe.a, stream_), "hipEventRecord");
+        }
+        dispatch(dtype_, [&](auto tt) { launch_tile_pass<decltype(tt)>(a, lds, stream_); });
+        if (prof_) {
+            hip_check(hipEventRecord(pe.b, stream_), "hipEventRecord");
+            pending_.push_back(pe);
+        }
+    }
+    hip_check(hipGetLastError(), "tile pass launch");
+    if (nslots > 0) {
+        slots_.ensure(static_cast<size_t>(nslots) * sizeof(double2));
+        launch_reduce_slots(static_cast<const double2 *>(partials_.p),
+                            reinterpret_cast<const SlotRange *>(dp + ranges_at), static_cast<uint32_t>(nslots),
+                            static_cast<double2 *>(slots_.p), stream_);
+        std::vector<double2> h(static_cast<size_t>(nslots));
+        hip_check(hipMemcpyAsync(h.data(), slots_.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_),
+                  "hipMemcpyAsync(slots)");
+        hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        for (int i = 0; i < nslots; ++i) result[i] = cd(h[i].x, h[i].y);
+    }
+    return result;
+}
+
+// ---------------------------------------------------------------------------
+void StateVector::apply_named(const std::string &name, const uint32_t *wires, int nw, const uint32_t *ctrls,
+                              const uint8_t *cvals, int nc, bool inverse, const double *params, int np) {
+    GateId g;
+    LQ_ABORT_IF(!gate_from_name(name, &g), (std::string("Operation does not exist for ") + name).c_str());
+    if (g == GateId::Identity) return;
+    const GateInfo &info = gate_info(g);
+    if (nc > 0)
+        LQ_ABORT_IF(!info.controllable, (std::string("Controlled operation does not exist for ") + name).c_str());
+    if (info.nwires > 0)
+        LQ_ABORT_IF(nw != info.nwires, (std::string("Wrong number of wires for ") + name).c_str());
+    else
+        LQ_ABORT_IF(nw < 1, "Number of wires must be larger than 0");
+    LQ_ABORT_IF(np < info.nparams, (std::string("Wrong number of parameters for ") + name).c_str());
+    check_wires(n_, wires, nw, ctrls, nc);
+    enqueue(lower_gate(n_, g, wires, nw, ctrls, cvals, nc, inverse, params));
+}
+
+void StateVector::apply_matrix(const void *mat, const uint32_t *wires, int nw, const uint32_t *ctrls,
+                               const uint8_t *cvals, int nc, bool inverse) {
+    LQ_ABORT_IF(nw < 1, "Number of wires must be larger than 0");
+    LQ_ABORT_IF(nw > 10, "dense matrices are supported on at most 10 wires");
+    check_wires(n_, wires, nw, ctrls, nc);
+    const size_t d = static_cast<size_t>(1) << nw;
+    std::vector<cd> m(d * d);
+    if (dtype_) {
+        const auto *src = static_cast<const std::complex<double> *>(mat);
+        for (size_t i = 0; i < d * d; ++i) m[i] = src[i];
+    } else {
+        const auto *src = static_cast<const std::complex<float> *>(mat);
+        for (size_t i = 0; i < d * d; ++i) m[i] = cd(src[i].real(), src[i].imag());
+    }
+    if (inverse) {
+        std::vector<cd> t(d * d);
+        for (size_t r = 0; r < d; ++r)
+            for (size_t c = 0; c < d; ++c) t[c * d + r] = std::conj(m[r * d + c]);
+        m.swap(t);
+    }
+    std::vector<int> tb(nw);
+    for (int j = 0; j < nw; ++j) tb[j] = phys_bit(n_, wires[j]);
+    uint64_t cm = 0, cv = 0;
+    for (int i = 0; i < nc; ++i) {
+        cm |= 1ull << phys_bit(n_, ctrls[i]);
+        if (!cvals || cvals[i]) cv |= 1ull << phys_bit(n_, ctrls[i]);
+    }
+    enqueue(lower_matrix(std::move(m), tb, cm, cv, false));
+}
+
+void StateVector::apply_pauli_rot(const uint32_t *wires, int nw, bool inverse, double theta, const std::string &word) {
+    LQ_ABORT_IF(static_cast<int>(word.size()) != nw, "wires and word have incompatible dimensions.");
+    check_wires(n_, wires, nw, nullptr, 0);
+    // exp(-i theta/2 P): an all-Z word is MultiRZ; any letter other than Z
+    // and Y flips its bit like X (GateImplementationsLM.hpp:575-629)
+    if (std::all_of(word.begin(), word.end(), [](char ch) { return ch == 'Z'; })) {
+        enqueue(lower_gate(n_, GateId::MultiRZ, wires, nw, nullptr, nullptr, 0, inverse, &theta));
+        return;
+    }
+    PauliWord w{1.0, 0, 0, 0};
+    for (int j = 0; j < nw; ++j) {
+        const uint64_t b = 1ull << phys_bit(n_, wires[j]);
+        if (word[j] == 'Z') w.yz |= b;
+        else if (word[j] == 'Y') {
+            w.x |= b;
+            w.yz |= b;
+            ++w.ny;
+        } else w.x |= b;
+    }
+    // cos(t/2) I - i sin(t/2) P as a dense op on the word's bits
+    std::vector<int> tb;
+    for (int j = 0; j < nw; ++j) tb.push_back(phys_bit(n_, wires[j]));
+    LQ_ABORT_IF(nw > 10, "PauliRot is supported on at most 10 wires");
+    const size_t d = static_cast<size_t>(1) << nw;
+    std::vector<cd> m(d * d, 0.0);
+    const double t = inverse ? -theta : theta;
+    const double c = std::cos(t / 2), s = std::sin(t / 2);
+    cd ph = 1.0;
+    for (int i = 0; i < (w.ny & 3); ++i) ph *= cd(0, -1);
+    for (size_t col = 0; col < d; ++col) {
+        m[col * d + col] += c;
+        // P|col>: flip x bits, sign from y/z bits of the row index
+        size_t row = col;
+        for (int j = 0; j < nw; ++j)
+            if (word[j] != 'Z') row ^= static_cast<size_t>(1) << (nw - 1 - j);
+        int sg = 0;
+        for (int j = 0; j < nw; ++j)
+            if ((word[j] == 'Y' || word[j] == 'Z') && ((row >> (nw - 1 - j)) & 1)) sg ^= 1;
+        m[row * d + col] += cd(0, -s) * ph * (sg ? -1.0 : 1.0);
+    }
+    enqueue(lower_matrix(std::move(m), tb, 0, 0, false));
+}
+
+double StateVector::apply_generator(const std::string &name, const uint32_t *wires, int nw, const uint32_t *ctrls,
+                                    const uint8_t *cvals, int nc) {
+    GateId g;
+    LQ_ABORT_IF(!gate_from_name(name, &g), (std::string("Generator does not exist for ") + name).c_str());
+    LQ_ABORT_IF(!gate_info(g).has_generator, (std::string("Generator does not exist for ") + name).c_str());
+    check_wires(n_, wires, g == GateId::GlobalPhase ? 0 : nw, ctrls, nc);
+    double scale = 0.0;
+    LOp op = lower_generator(n_, g, wires, nw, ctrls, cvals, nc, &scale);
+    if (!(g == GateId::GlobalPhase && nc == 0)) enqueue(std::move(op));
+    return scale;
+}
+
+void StateVector::apply_oprec(const OpRec &op, bool adj) {
+    const bool inv = op.inverse ^ adj;
+    const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
+    const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
+    if (op.named) {
+        apply_named(op.name, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
+                    static_cast<int>(op.ctrls.size()), inv, op.params.data(), static_cast<int>(op.params.size()));
+        return;
+    }
+    LQ_ABORT_IF(!op.ctrls.empty(), (std::string("Controlled operation does not exist for ") + op.name).c_str());
+    LQ_ABORT_IF(op.matrix.size() != (static_cast<size_t>(1) << (2 * op.wires.size())),
+                (std::string("Operation does not exist for ") + op.name).c_str());
+    check_wires(n_, op.wires.data(), static_cast<int>(op.wires.size()), nullptr, 0);
+    std::vector<cd> m = op.matrix;
+    const size_t d = static_cast<size_t>(1) << op.wires.size();
+    if (inv) {
+        std::vector<cd> t(d * d);
+        for (size_t r = 0; r < d; ++r)
+            for (size_t c = 0; c < d; ++c) t[c * d + r] = std::conj(m[r * d + c]);
+        m.swap(t);
+    }
+    std::vector<int> tb;
+    for (auto w : op.wires) tb.push_back(phys_bit(n_, w));
+    enqueue(lower_matrix(std::move(m), tb, 0, 0, false));
+}
+
+double StateVector::norm2() {
+    flush();
+    const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
+    partials_.ensure(nb * sizeof(double2));
+    dispatch(dtype_, [&](auto t) {
+        launch_norm2<decltype(t)>(d_, n_, static_cast<double2 *>(partials_.p), nb, stream_);
+    });
+    std::vector<double2> h(nb);
+    hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    double s = 0.0, c = 0.0;
+    for (const auto &v : h) {
+        const double t = s + v.x;
+        c += (std::fabs(s) >= std::fabs(v.x)) ? (s - t) + v.x : (v.x - t) + s;
+        s = t;
+    }
+    return s + c;
+}
+
+void StateVector::normalize() {
+    const double norm = std::sqrt(norm2());
+    const double eps = dtype_ ? std::numeric_limits<double>::epsilon() : std::numeric_limits<float>::epsilon();
+    LQ_ABORT_IF(norm < eps * 1e2, "Vector has norm close to zero and cannot be normalized");
+    dispatch(dtype_, [&](auto t) { launch_scale<decltype(t)>(d_, n_, 1.0 / norm, 0.0, stream_); });
+}
+
+void StateVector::collapse(uint32_t wire, bool branch) {
+    LQ_ABORT_IF(static_cast<int>(wire) >= n_, "wire index out of range");
+    flush();
+    dispatch(dtype_, [&](auto t) {
+        launch_collapse<decltype(t)>(d_, n_, phys_bit(n_, wire), branch ? 1 : 0, 1.0, stream_);
+    });
+    normalize();
+}
+
+std::vector<double> StateVector::reduce_rows_to_host(uint32_t rows, uint32_t cols) {
+    slots_.ensure(static_cast<size_t>(cols) * sizeof(double));
+    launch_reduce_rows(static_cast<const double *>(partials_.p), rows, cols, static_cast<double *>(slots_.p), stream_);
+    std::vector<double> h(cols);
+    hip_check(hipMemcpyAsync(h.data(), slots_.p, cols * sizeof(double), hipMemcpyDeviceToHost, stream_), "memcpy");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return h;
+}
+
+std::vector<double> StateVector::probs(const uint32_t *wires, int nw, bool all) {
+    if (!all && nw == 0) return {1.0};
+    std::vector<int> bits;
+    if (all) {
+        for (int b = n_ - 1; b >= 0; --b) bits.push_back(b);
+    } else {
+        check_wires(n_, wires, nw, nullptr, 0);
+        for (int j = 0; j < nw; ++j) bits.push_back(phys_bit(n_, wires[j]));
+    }
+    const int m = static_cast<int>(bits.size());
+    flush();
+    if (m > 12) {
+        slots_.ensure(sizeof(double) << m);
+        dispatch(dtype_, [&](auto t) {
+            launch_probs_gather<decltype(t)>(d_, n_, bits.data(), m, static_cast<double *>(slots_.p), stream_);
+        });
+        std::vector<double> h(static_cast<size_t>(1) << m);
+        hip_check(hipMemcpyAsync(h.data(), slots_.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_),
+                  "memcpy");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        return h;
+    }
+    const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(512, std::max<uint64_t>(1, length() / 1024)));
+    partials_.ensure((static_cast<size_t>(nb) * sizeof(double)) << m);
+    dispatch(dtype_, [&](auto t) {
+        launch_probs_hist<decltype(t)>(d_, n_, bits.data(), m, static_cast<double *>(partials_.p), nb, stream_);
+    });
+    return reduce_rows_to_host(nb, 1u << m);
+}
+
+double StateVector::expval_named(const std::string &name, const uint32_t *wires, int nw) {
+    if (name == "Identity") return 1.0;
+    const auto &m = named_obs_matrix(name);
+    LQ_ABORT_IF(nw != 1, "named observables act on one wire");
+    check_wires(n_, wires, 1, nullptr, 0);
+    flush();
+    LOp op = make_dot(lower_matrix(m, {phys_bit(n_, wires[0])}, 0, 0, true), 0, 0, 0);
+    return run({op}, {d_}, 1)[0].real();
+}
+
+std::vector<double> StateVector::expval_named_batch(const std::vector<std::string> &names,
+                                                    const std::vector<uint32_t> &wires) {
+    LQ_ABORT_IF(names.size() != wires.size(), "names and wires differ in length");
+    flush();
+    std::vector<LOp> prog;
+    std::vector<int> slot_of(names.size(), -1);
+    for (size_t i = 0; i < names.size(); ++i) {
+        if (names[i] == "Identity") continue;
+        const auto &m = named_obs_matrix(names[i]);
+        check_wires(n_, &wires[i], 1, nullptr, 0);
+        slot_of[i] = static_cast<int>(prog.size());
+        prog.push_back(make_dot(lower_matrix(m, {phys_bit(n_, wires[i])}, 0, 0, true), 0, 0, slot_of[i]));
+    }
+    const auto v = run(prog, {d_}, static_cast<int>(prog.size()));
+    std::vector<double> out(names.size(), 1.0);
+    for (size_t i = 0; i < names.size(); ++i)
+        if (slot_of[i] >= 0) out[i] = v[slot_of[i]].real();
+    return out;
+}
+
+double StateVector::expval_matrix(const void *mat, const uint32_t *wires, int nw) {
+    LQ_ABORT_IF(nw < 1 || nw > 10, "matrix observables are supported on 1 to 10 wires");
+    check_wires(n_, wires, nw, nullptr, 0);
+    const size_t d = static_cast<size_t>(1) << nw;
+    std::vector<cd> m(d * d);
+    if (dtype_) {
+        const auto *s = static_cast<const std::complex<double> *>(mat);
+        for (size_t i = 0; i < d * d; ++i) m[i] = s[i];
+    } else {
+        const auto *s = static_cast<const std::complex<float> *>(mat);
+        for (size_t i = 0; i < d * d; ++i) m[i] = cd(s[i].real(), s[i].imag());
+    }
+    std::vector<int> tb(nw);
+    for (int j = 0; j < nw; ++j) tb[j] = phys_bit(n_, wires[j]);
+    flush();
+    LOp op = make_dot(lower_matrix(std::move(m), tb, 0, 0, true), 0, 0, 0);
+    return run({op}, {d_}, 1)[0].real();
+}
+
+cd StateVector::inner(const void *bra, const void *ket) {
+    return run({identity_dot(1, 0, 0)}, {const_cast<void *>(bra), const_cast<void *>(ket)}, 1)[0];
+}
+
+void StateVector::apply_obs_inplace(const Obs &o, void *state, std::vector<LOp> &prog) {
+    (void)state;
+    switch (o.kind) {
+    case Obs::Named: {
+        if (o.name == "Identity") return;
+        GateId g;
+        std::vector<cd> m;
+        if (o.name == "PauliX" || o.name == "PauliY" || o.name == "PauliZ" || o.name == "Hadamard") {
+            m = named_obs_matrix(o.name);
+        } else {
+            LQ_ABORT_IF(!gate_from_name(o.name, &g), (std::string("Operation does not exist for ") + o.name).c_str());
+            LQ_ABORT_IF(gate_info(g).nparams != 0, "parametric named observables are not supported");
+            m = gate_matrix(g, nullptr, false, static_cast<int>(o.wires.size()));
+        }
+        std::vector<int> tb;
+        for (auto w : o.wires) tb.push_back(phys_bit(n_, w));
+        prog.push_back(lower_matrix(std::move(m), tb, 0, 0, false));
+        return;
+    }
+    case Obs::Hermitian: {
+        std::vector<int> tb;
+        for (auto w : o.wires) tb.push_back(phys_bit(n_, w));
+        prog.push_back(lower_matrix(o.matrix, tb, 0, 0, false));
+        return;
+    }
+    case Obs::Tensor:
+        for (const auto &f : o.terms) apply_obs_inplace(*f, state, prog);
+        return;
+    case Obs::Ham: LQ_ABORT("a Hamiltonian cannot be applied in place as a product factor");
+    }
+}
+
+void StateVector::apply_obs_to(const Obs &o, const void *in, void *out) {
+    for (auto w : o.all_wires()) LQ_ABORT_IF(static_cast<int>(w) >= n_, "observable wire out of range");
+    std::vector<PauliWord> words;
+    if (obs_to_pauli(o, n_, &words)) {
+        std::vector<PauliTerm> terms;
+        for (const auto &w : words) {
+            cd c = w.coeff;
+            for (int i = 0; i < (w.ny & 3); ++i) c *= cd(0, -1);
+            terms.push_back({w.x, w.yz, c.real(), c.imag()});
+        }
+        small_.ensure(terms.size() * sizeof(PauliTerm) + 256);
+        hip_check(hipMemcpyAsync(small_.p, terms.data(), terms.size() * sizeof(PauliTerm), hipMemcpyHostToDevice,
+                                 stream_),
+                  "memcpy(terms)");
+        dispatch(dtype_, [&](auto t) {
+            launch_pauli_sum<decltype(t)>(in, out, static_cast<const PauliTerm *>(small_.p),
+                                          static_cast<uint32_t>(terms.size()), n_, 1, nullptr, 0, stream_);
+        });
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        return;
+    }
+    if (o.kind == Obs::Ham) {
+        void *tmp = workspace(7);
+        LQ_ABORT_IF(tmp == in || tmp == out, "workspace clash");
+        // a fresh workspace may hold NaN bit patterns: clear it, do not scale it
+        hip_check(hipMemsetAsync(out, 0, bytes(), stream_), "hipMemsetAsync");
+        for (size_t i = 0; i < o.terms.size(); ++i) {
+            hip_check(hipMemcpyAsync(tmp, in, bytes(), hipMemcpyDeviceToDevice, stream_), "memcpy");
+            if (o.terms[i]->kind == Obs::Ham) {
+                LQ_ABORT("nested Hamiltonians are not supported");
+            }
+            std::vector<LOp> prog;
+            apply_obs_inplace(*o.terms[i], tmp, prog);
+            run(prog, {tmp}, 0);
+            dispatch(dtype_, [&](auto t) {
+                launch_axpy<decltype(t)>(tmp, out, o.coeffs[i], 0.0, n_, stream_);
+            });
+        }
+        return;
+    }
+    hip_check(hipMemcpyAsync(out, in, bytes(), hipMemcpyDeviceToDevice, stream_), "memcpy");
+    std::vector<LOp> prog;
+    apply_obs_inplace(o, out, prog);
+    run(prog, {out}, 0);
+}
+
+double StateVector::expval_obs(const Obs &o) {
+    for (auto w : o.all_wires()) LQ_ABORT_IF(static_cast<int>(w) >= n_, "observable wire out of range");
+    flush();
+    std::vector<PauliWord> words;
+    if (obs_to_pauli(o, n_, &words)) {
+        const int ktm = std::min(kt_for(1), n_);
+        bool fits = words.size() <= 32;
+        for (const auto &w : words) fits = fits && __builtin_popcountll(w.x) <= ktm - 2;
+        if (fits) {
+            std::vector<LOp> prog;
+            for (size_t i = 0; i < words.size(); ++i) prog.push_back(make_dot(pauli_op(words[i]), 0, 0, static_cast<int>(i)));
+            const auto v = run(prog, {d_}, static_cast<int>(prog.size()));
+            double e = 0.0;
+            for (size_t i = 0; i < words.size(); ++i) e += (words[i].coeff * v[i]).real();
+            return e;
+        }
+        std::vector<PauliTerm> terms;
+        for (const auto &w : words) {
+            cd c = w.coeff;
+            for (int i = 0; i < (w.ny & 3); ++i) c *= cd(0, -1);
+            terms.push_back({w.x, w.yz, c.real(), c.imag()});
+        }
+        small_.ensure(terms.size() * sizeof(PauliTerm) + 256);
+        hip_check(hipMemcpyAsync(small_.p, terms.data(), terms.size() * sizeof(PauliTerm), hipMemcpyHostToDevice,
+                                 stream_),
+                  "memcpy(terms)");
+        const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
+        partials_.ensure(nb * sizeof(double2));
+        dispatch(dtype_, [&](auto t) {
+            launch_pauli_sum<decltype(t)>(d_, nullptr, static_cast<const PauliTerm *>(small_.p),
+                                          static_cast<uint32_t>(terms.size()), n_, 0,
+                                          static_cast<double2 *>(partials_.p), nb, stream_);
+        });
+        std::vector<double2> h(nb);
+        hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        double s = 0.0, c = 0.0;
+        for (const auto &v : h) {
+            const double t = s + v.x;
+            c += (std::fabs(s) >= std::fabs(v.x)) ? (s - t) + v.x : (v.x - t) + s;
+            s = t;
+        }
+        return s + c;
+    }
+    void *tmp = workspace(6);
+    apply_obs_to(o, d_, tmp);
+    return inner(d_, tmp).real();
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint Jacobian (lq/algorithms/AdjointJacobianLQubit.hpp:225-491)
+// ---------------------------------------------------------------------------
+void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std::shared_ptr<Obs>> &obs,
+                      const std::vector<uint64_t> &tp_in, bool apply_operations, double *jac, double *expvals) {
+    std::vector<uint64_t> tp = tp_in;
+    std::sort(tp.begin(), tp.end());
+    const size_t ntp = tp.size(), nobs = obs.size();
+    if (ntp == 0) return;
+    LQ_ABORT_IF(nobs == 0, "no observables given");
+    const int n = sv.n();
+    sv.flush();
+    // lambda, kept unchanged across observable chunks as workspace 0
+    void *lam0 = sv.workspace(0);
+    hip_check(hipMemcpyAsync(lam0, sv.data(), sv.bytes(), hipMemcpyDeviceToDevice, sv.stream()), "memcpy");
+    if (apply_operations) {
+        std::vector<LOp> fwd;
+        for (const auto &op : ops.ops) {
+            if (op.name == "Identity") continue;
+            // reuse StateVector lowering through a temporary queue on lambda
+            const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
+            const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
+            if (op.named) {
+                fwd.push_back(lower_gate(n, op.gid, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
+                                         static_cast<int>(op.ctrls.size()), op.inverse, op.params.data()));
+            } else {
+                std::vector<cd> m = op.matrix;
+                const size_t d = static_cast<size_t>(1) << op.wires.size();
+                if (op.inverse) {
+                    std::vector<cd> t(d * d);
+                    for (size_t r = 0; r < d; ++r)
+                        for (size_t c = 0; c < d; ++c) t[c * d + r] = std::conj(m[r * d + c]);
+                    m.swap(t);
+                }
+                std::vector<int> tb;
+                for (auto w : op.wires) tb.push_back(phys_bit(n, w));
+                fwd.push_back(lower_matrix(std::move(m), tb, 0, 0, false));
+            }
+        }
+        sv.run(fwd, {lam0}, 0);
+    }
+    // one H|lambda> per sweep keeps two tiles in LDS; more observables re-run
+    // the sweep from a saved lambda
+    const size_t per_sweep = 1;
+    std::vector<double> out(nobs * ntp, 0.0);
+    for (size_t o0 = 0; o0 < nobs; o0 += per_sweep) {
+        const size_t m = std::min(per_sweep, nobs - o0);
+        void *lam = lam0;
+        if (nobs > per_sweep) {
+            lam = sv.workspace(1);
+            hip_check(hipMemcpyAsync(lam, lam0, sv.bytes(), hipMemcpyDeviceToDevice, sv.stream()), "memcpy");
+        }
+        std::vector<void *> states{lam};
+        for (size_t j = 0; j < m; ++j) {
+            void *h = sv.workspace(2 + static_cast<int>(j));
+            sv.apply_obs_to(*obs[o0 + j], lam, h);
+            states.push_back(h);
+        }
+        const uint32_t all = (1u << states.size()) - 1u;
+        std::vector<LOp> prog;
+        std::vector<double> scales(ntp, 0.0);
+        int nslot = 0;
+        if (expvals) {
+            for (size_t j = 0; j < m; ++j) prog.push_back(identity_dot(0, 1 + static_cast<uint32_t>(j), nslot++));
+        }
+        const int exp_slots = nslot;
+        size_t tp_pos = ntp; // tp[tp_pos-1] is the next trainable index going backwards
+        int64_t cur = static_cast<int64_t>(ops.num_par_ops) - 1;
+        int64_t tnum = static_cast<int64_t>(ntp) - 1;
+        size_t last_dot = 0;
+        for (int64_t k = static_cast<int64_t>(ops.ops.size()) - 1; k >= 0; --k) {
+            const OpRec &op = ops.ops[static_cast<size_t>(k)];
+            LQ_ABORT_IF(op.params.size() > 1, "The operation is not supported using the adjoint differentiation method");
+            if (op.name == "StatePrep" || op.name == "BasisState") continue;
+            if (tp_pos == 0) break;
+            if (!op.params.empty()) {
+                if (cur == static_cast<int64_t>(tp[tp_pos - 1])) {
+                    LQ_ABORT_IF(!op.named || !gate_info(op.gid).has_generator,
+                                (std::string("The operation is not supported using the adjoint differentiation method: ") +
+                                 op.name).c_str());
+                    double s = 0.0;
+                    const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
+                    const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
+                    LOp g = lower_generator(n, op.gid, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
+                                            static_cast<int>(op.ctrls.size()), &s);
+                    scales[static_cast<size_t>(tnum)] = s * (op.inverse ? -1.0 : 1.0);
+                    for (size_t j = 0; j < m; ++j)
+                        prog.push_back(make_dot(g, 0, 1 + static_cast<uint32_t>(j),
+                                                exp_slots + static_cast<int>(tnum * static_cast<int64_t>(m) + j)));
+                    last_dot = prog.size();
+                    --tnum;
+                    --tp_pos;
+                }
+                --cur;
+            }
+            // U^dagger on lambda and every H lambda
+            LOp u;
+            const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
+            const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
+            if (op.named) {
+                if (op.gid == GateId::Identity) continue;
+                u = lower_gate(n, op.gid, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
+                               static_cast<int>(op.ctrls.size()), !op.inverse, op.params.data());
+            } else {
+                LQ_ABORT_IF(!op.ctrls.empty(), (std::string("Controlled operation does not exist for ") + op.name).c_str());
+                std::vector<cd> mm = op.matrix;
+                const size_t d = static_cast<size_t>(1) << op.wires.size();
+                LQ_ABORT_IF(mm.size() != d * d, (std::string("Operation does not exist for ") + op.name).c_str());
+                if (!op.inverse) {
+                    std::vector<cd> t(d * d);
+                    for (size_t r = 0; r < d; ++r)
+                        for (size_t c = 0; c < d; ++c) t[c * d + r] = std::conj(mm[r * d + c]);
+                    mm.swap(t);
+                }
+                std::vector<int> tb;
+                for (auto w : op.wires) tb.push_back(phys_bit(n, w));
+                u = lower_matrix(std::move(mm), tb, 0, 0, false);
+            }
+            u.smask = all;
+            prog.push_back(std::move(u));
+        }
+        // ops after the last dot change no result
+        prog.resize(std::max(last_dot, static_cast<size_t>(exp_slots)));
+        const int nslots = exp_slots + static_cast<int>(ntp * m);
+        const auto v = sv.run(prog, states, nslots);
+        for (size_t j = 0; j < m; ++j) {
+            if (expvals) expvals[o0 + j] = v[j].real();
+            for (size_t t = 0; t < ntp; ++t)
+                out[(o0 + j) * ntp + t] = -2.0 * scales[t] * v[exp_slots + t * m + j].imag();
+        }
+    }
+    std::copy(out.begin(), out.end(), jac);
+}
+
+} // namespace lqamd

@@ -1,0 +1,184 @@
+// Device state vector, tile-program executor, measurements, observables and
+// the adjoint Jacobian on one MI355X.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "lq_common.hpp"
+#include "lq_gates.hpp"
+#include "lq_kernels.hpp"
+#include "lq_plan.hpp"
+
+namespace lqamd {
+
+void hip_check(hipError_t e, const char *what);
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    void ensure(size_t b);
+    void release();
+};
+
+struct HostPinned {
+    void *p = nullptr;
+    size_t bytes = 0;
+    HostPinned() = default;
+    HostPinned(const HostPinned &) = delete;
+    HostPinned &operator=(const HostPinned &) = delete;
+    ~HostPinned();
+    void ensure(size_t b);
+};
+
+// Observables (core/observables/Observables.hpp:128-584)
+struct Obs {
+    enum Kind { Named, Hermitian, Tensor, Ham } kind = Named;
+    std::string name;
+    std::vector<uint32_t> wires;
+    std::vector<cd> matrix;                 // Hermitian, row-major complex128
+    std::vector<std::shared_ptr<Obs>> terms; // Tensor factors / Hamiltonian terms
+    std::vector<double> coeffs;             // Hamiltonian
+    std::vector<uint32_t> all_wires() const;
+};
+
+struct PauliWord {
+    cd coeff;
+    uint64_t x = 0, yz = 0; // physical bit masks
+    int ny = 0;
+};
+// Expand into a Pauli sum on physical bits; false if a Hermitian factor occurs.
+bool obs_to_pauli(const Obs &o, int n, std::vector<PauliWord> *out);
+
+// One entry of OpsData (core/algorithms/JacobianData.hpp:39-253)
+struct OpRec {
+    std::string name;
+    bool named = false;
+    GateId gid = GateId::Identity;
+    std::vector<double> params;
+    std::vector<uint32_t> wires;
+    bool inverse = false;
+    std::vector<cd> matrix;
+    std::vector<uint32_t> ctrls;
+    std::vector<uint8_t> cvals;
+};
+struct OpsList {
+    std::vector<OpRec> ops;
+    size_t num_par_ops = 0;
+};
+
+struct EngineTuning {
+    int kt_max = 0;      // 0: from the LDS budget
+    int low_bits = -1;   // -1: 256-byte segments
+    double budget = 24.0;
+    size_t lds_bytes = 65536;
+};
+EngineTuning &tuning();
+
+class StateVector {
+  public:
+    StateVector(int n, int dtype);
+    ~StateVector();
+    StateVector(const StateVector &) = delete;
+    StateVector &operator=(const StateVector &) = delete;
+
+    int n() const { return n_; }
+    int dtype() const { return dtype_; } // 0: complex64, 1: complex128
+    size_t elem() const { return dtype_ ? 16 : 8; }
+    uint64_t length() const { return 1ull << n_; }
+    size_t bytes() const { return elem() << n_; }
+    void *data() { return d_; }
+    hipStream_t stream() const { return stream_; }
+
+    // state preparation (lq/StateVectorLQubit.hpp:928-1036)
+    void reset();
+    void set_basis_index(uint64_t index);
+    void set_basis_state(const uint32_t *bits, const uint32_t *wires, int nw);
+    void set_state_vector(const void *host, const uint32_t *wires, int nw);
+    void update_data(const void *host, uint64_t len);
+    void get_data(void *host, uint64_t len);
+    void copy_from(StateVector &src);
+
+    // gates (lq/StateVectorLQubit.hpp:378-710)
+    void apply_named(const std::string &name, const uint32_t *wires, int nw, const uint32_t *ctrls,
+                     const uint8_t *cvals, int nc, bool inverse, const double *params, int np);
+    void apply_matrix(const void *mat, const uint32_t *wires, int nw, const uint32_t *ctrls, const uint8_t *cvals,
+                      int nc, bool inverse);
+    void apply_pauli_rot(const uint32_t *wires, int nw, bool inverse, double theta, const std::string &word);
+    double apply_generator(const std::string &name, const uint32_t *wires, int nw, const uint32_t *ctrls,
+                           const uint8_t *cvals, int nc);
+    void apply_oprec(const OpRec &op, bool adj);
+    void normalize();
+    void collapse(uint32_t wire, bool branch);
+
+    // measurements (lq/measurements/MeasurementsLQubit.hpp)
+    std::vector<double> probs(const uint32_t *wires, int nw, bool all);
+    double expval_named(const std::string &name, const uint32_t *wires, int nw);
+    std::vector<double> expval_named_batch(const std::vector<std::string> &names, const std::vector<uint32_t> &wires);
+    double expval_matrix(const void *mat, const uint32_t *wires, int nw);
+    double expval_obs(const Obs &o);
+    double norm2();
+
+    // queue
+    void enqueue(LOp op);
+    void flush();
+    void synchronize();
+
+    // Run an op program over `states` (device pointers of this size/dtype),
+    // return the dot slot values (complex) indexed by slot id.
+    std::vector<cd> run(const std::vector<LOp> &prog, const std::vector<void *> &states, int nslots);
+
+    // out = O in (observable applied, Hamiltonians summed); in/out device states
+    void apply_obs_to(const Obs &o, const void *in, void *out);
+    cd inner(const void *bra, const void *ket);
+
+    // Per-pass timing with HIP events on this handle's stream: when enabled,
+    // every tile-pass launch is bracketed by events; read() totals them.
+    struct PassStats {
+        uint64_t passes = 0;
+        double ms = 0.0;
+        double bytes = 0.0; // algorithmic HBM bytes (amplitudes read + written)
+        double max_ms = 0.0;
+    };
+    void profile(bool on);
+    PassStats profile_read(bool reset);
+
+    // workspace states (kept between calls; released with release_workspace)
+    void *workspace(int i);
+    void release_workspace();
+
+  private:
+    int n_, dtype_;
+    void *d_ = nullptr;
+    hipStream_t stream_ = nullptr;
+    std::vector<LOp> queue_;
+    DevBuf prog_, partials_, slots_, small_;
+    HostPinned staging_;
+    hipEvent_t staged_ = nullptr;
+    std::vector<std::unique_ptr<DevBuf>> ws_;
+    bool prof_ = false;
+    struct Pending {
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> ev_free_;
+    PassStats stats_;
+    hipEvent_t take_event();
+
+    int kt_for(int ns) const;
+    std::vector<double> reduce_rows_to_host(uint32_t rows, uint32_t cols);
+    void apply_obs_inplace(const Obs &o, void *state, std::vector<LOp> &prog);
+};
+
+void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std::shared_ptr<Obs>> &obs,
+                      const std::vector<uint64_t> &tp, bool apply_operations, double *jac, double *expvals);
+
+} // namespace lqamd

@@ -1,0 +1,735 @@
+// HIP kernels for gfx950 (MI355X): the LDS tile-program kernel that applies
+// runs of gates (and reads generator inner products) in one HBM pass, plus
+// the standalone reductions and fills around it.
+//
+// Amplitude layout: interleaved complex (double2 / float2), index bit n-1-w
+// is wire w (lq/StateVectorLQubit.hpp:950). All sums are accumulated in
+// double, also for complex64 states.
+#include <hip/hip_runtime.h>
+
+#include "lq_kernels.hpp"
+
+namespace lqamd {
+namespace {
+
+template <typename T> struct VecT;
+template <> struct VecT<double> { using type = double2; };
+template <> struct VecT<float> { using type = float2; };
+
+template <typename T> __device__ __forceinline__ typename VecT<T>::type mk(T x, T y) {
+    typename VecT<T>::type v;
+    v.x = x;
+    v.y = y;
+    return v;
+}
+
+template <typename T> __device__ __forceinline__ typename VecT<T>::type ld_coef(const double2 *c, uint64_t i) {
+    const double2 d = c[i];
+    return mk<T>(static_cast<T>(d.x), static_cast<T>(d.y));
+}
+
+// a*b
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+    V r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+// a*b + c*d
+template <typename V> __device__ __forceinline__ V cmul2(V a, V b, V c, V d) {
+    V r;
+    r.x = a.x * b.x - a.y * b.y + c.x * d.x - c.y * d.y;
+    r.y = a.x * b.y + a.y * b.x + c.x * d.y + c.y * d.x;
+    return r;
+}
+// acc += conj(b) * v, in double
+template <typename V> __device__ __forceinline__ void acc_dot(double &re, double &im, V b, V v) {
+    const double bx = b.x, by = b.y, vx = v.x, vy = v.y;
+    re += bx * vx + by * vy;
+    im += bx * vy - by * vx;
+}
+
+__device__ __forceinline__ uint32_t insert_zero(uint32_t p, int q) {
+    return ((p >> q) << (q + 1)) | (p & ((1u << q) - 1u));
+}
+
+__device__ __forceinline__ void block_reduce_store(double re, double im, double2 *red, double2 *dst) {
+    for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off, 64);
+        im += __shfl_xor(im, off, 64);
+    }
+    const uint32_t tid = threadIdx.x;
+    if ((tid & 63u) == 0) red[tid >> 6] = make_double2(re, im);
+    __syncthreads();
+    if (tid == 0) {
+        double2 s = red[0];
+        for (int w = 1; w < kBlock / 64; ++w) {
+            s.x += red[w].x;
+            s.y += red[w].y;
+        }
+        *dst = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile-program kernel
+// ---------------------------------------------------------------------------
+template <typename T, int NS>
+__global__ __launch_bounds__(kBlock) void tile_pass_kernel(TilePassArgs a) {
+    using V = typename VecT<T>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V *lds = reinterpret_cast<V *>(smem_raw);
+    __shared__ double2 red[kBlock / 64];
+
+    const uint32_t tid = threadIdx.x;
+    const uint64_t tile = static_cast<uint64_t>(blockIdx.x) + static_cast<uint64_t>(blockIdx.y) * gridDim.x;
+    if (tile >= a.ntiles) return; // uniform over the block
+    const int kt = a.g.kt;
+    const int c = a.g.c;
+    const uint32_t tsize = 1u << kt;
+
+    uint64_t base = 0;
+    for (int j = 0; j < a.g.nout; ++j) base |= ((tile >> j) & 1ull) << a.g.ob[j];
+
+    // spread(l) = base offset of tile-local index l; separable over the bits
+    // of l, so spread(tid + it*kBlock) = sp_tid | spread(it*kBlock).
+    auto spread = [&](uint32_t l) -> uint64_t {
+        uint64_t s = l & ((1u << c) - 1u);
+        for (int b = c; b < kt; ++b) s |= static_cast<uint64_t>((l >> b) & 1u) << a.g.tb[b];
+        return s;
+    };
+    const uint64_t sp_tid = spread(tid < tsize ? tid : 0u);
+    const uint32_t iters = (tsize + kBlock - 1) / kBlock;
+
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (!((a.load_mask >> s) & 1u)) continue;
+        const V *src = reinterpret_cast<const V *>(a.sv[s]);
+        for (uint32_t it = 0; it < iters; ++it) {
+            const uint32_t l = tid + it * kBlock;
+            if (l < tsize) lds[s * tsize + l] = src[base | sp_tid | spread(it * kBlock)];
+        }
+    }
+    __syncthreads();
+
+    const V zero = mk<T>(T(0), T(0));
+    for (uint32_t oi = 0; oi < a.nops; ++oi) {
+        const TileOp &op = a.ops[oi];
+        const uint32_t kind = op.kind;
+        const bool dot = (op.flags & TF_DOT) != 0;
+        const bool zero_unctrl = (op.flags & TF_ZERO_UNCTRL) != 0;
+        const uint32_t smask = op.smask;
+        double2 *dst = dot ? &a.partials[static_cast<uint64_t>(op.slot) * a.ntiles + tile] : nullptr;
+
+        if ((base & op.ocmask) != op.ocval) {
+            // control bits outside the tile do not match: no-op for this tile
+            if (dot) {
+                if (tid == 0) *dst = make_double2(0.0, 0.0);
+            } else if (zero_unctrl) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    if ((smask >> s) & 1u)
+                        for (uint32_t l = tid; l < tsize; l += kBlock) lds[s * tsize + l] = zero;
+            }
+            __syncthreads();
+            continue;
+        }
+        const uint32_t lcm = op.lcmask, lcv = op.lcval;
+        const V *A = lds + op.sa * tsize;
+        const V *B = lds + op.sb * tsize;
+        double accr = 0.0, acci = 0.0;
+
+        if (kind == TOP_DENSE1) {
+            const int q = op.lpos[0];
+            const uint32_t qb = 1u << q;
+            const V m00 = ld_coef<T>(a.coef, op.coef_off + 0), m01 = ld_coef<T>(a.coef, op.coef_off + 1);
+            const V m10 = ld_coef<T>(a.coef, op.coef_off + 2), m11 = ld_coef<T>(a.coef, op.coef_off + 3);
+            for (uint32_t p = tid; p < (tsize >> 1); p += kBlock) {
+                const uint32_t i0 = insert_zero(p, q), i1 = i0 | qb;
+                const bool ok = (i0 & lcm) == lcv;
+                if (dot) {
+                    if (ok) {
+                        const V a0 = A[i0], a1 = A[i1];
+                        acc_dot(accr, acci, B[i0], cmul2(m00, a0, m01, a1));
+                        acc_dot(accr, acci, B[i1], cmul2(m10, a0, m11, a1));
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if (!((smask >> s) & 1u)) continue;
+                        V *S = lds + s * tsize;
+                        if (ok) {
+                            const V a0 = S[i0], a1 = S[i1];
+                            S[i0] = cmul2(m00, a0, m01, a1);
+                            S[i1] = cmul2(m10, a0, m11, a1);
+                        } else if (zero_unctrl) {
+                            S[i0] = zero;
+                            S[i1] = zero;
+                        }
+                    }
+                }
+            }
+        } else if (kind == TOP_DENSE2) {
+            const int q0 = op.lpos[0], q1 = op.lpos[1];
+            const int lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
+            const uint32_t b0 = 1u << q0, b1 = 1u << q1;
+            V m[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) m[e] = ld_coef<T>(a.coef, op.coef_off + e);
+            for (uint32_t p = tid; p < (tsize >> 2); p += kBlock) {
+                const uint32_t i00 = insert_zero(insert_zero(p, lo), hi);
+                const uint32_t idx[4] = {i00, i00 | b1, i00 | b0, i00 | b0 | b1};
+                const bool ok = (i00 & lcm) == lcv;
+                if (dot) {
+                    if (ok) {
+                        V v[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) v[e] = A[idx[e]];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const V o = cmul2(m[4 * r], v[0], m[4 * r + 1], v[1]);
+                            const V o2 = cmul2(m[4 * r + 2], v[2], m[4 * r + 3], v[3]);
+                            acc_dot(accr, acci, B[idx[r]], mk<T>(o.x + o2.x, o.y + o2.y));
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if (!((smask >> s) & 1u)) continue;
+                        V *S = lds + s * tsize;
+                        if (ok) {
+                            V v[4];
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) v[e] = S[idx[e]];
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const V o = cmul2(m[4 * r], v[0], m[4 * r + 1], v[1]);
+                                const V o2 = cmul2(m[4 * r + 2], v[2], m[4 * r + 3], v[3]);
+                                S[idx[r]] = mk<T>(o.x + o2.x, o.y + o2.y);
+                            }
+                        } else if (zero_unctrl) {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) S[idx[e]] = zero;
+                        }
+                    }
+                }
+            }
+        } else if (kind == TOP_DENSEK) {
+            const int k = static_cast<int>(op.k);
+            const uint32_t dim = 1u << k;
+            int srt[kMaxTileOpTargets];
+            for (int j = 0; j < k; ++j) srt[j] = op.lpos[j];
+            for (int i = 1; i < k; ++i) // insertion sort, uniform
+                for (int j = i; j > 0 && srt[j - 1] > srt[j]; --j) {
+                    const int t = srt[j];
+                    srt[j] = srt[j - 1];
+                    srt[j - 1] = t;
+                }
+            auto lsp = [&](uint32_t col) -> uint32_t {
+                uint32_t o = 0;
+                for (int j = 0; j < k; ++j) o |= ((col >> (k - 1 - j)) & 1u) << op.lpos[j];
+                return o;
+            };
+            const uint32_t R = dim > static_cast<uint32_t>(kBlock) ? dim / kBlock : 1u;
+            const uint32_t per_round = kBlock * R;
+            const uint32_t rounds = tsize > per_round ? tsize / per_round : 1u;
+            const double2 *M = a.coef + op.coef_off;
+            for (int s = 0; s < (dot ? 1 : NS); ++s) {
+                if (!dot && !((smask >> s) & 1u)) continue;
+                V *S = dot ? const_cast<V *>(A) : lds + s * tsize;
+                for (uint32_t r = 0; r < rounds; ++r) {
+                    V outv[4];
+                    uint32_t outi[4];
+                    bool outok[4];
+                    for (uint32_t rr = 0; rr < R; ++rr) {
+                        const uint32_t e = r * per_round + rr * kBlock + tid;
+                        outok[rr] = false;
+                        outi[rr] = 0;
+                        outv[rr] = zero;
+                        if (e >= tsize) continue;
+                        const uint32_t g = e >> k, row = e & (dim - 1u);
+                        uint32_t gb = g;
+                        for (int j = 0; j < k; ++j) gb = insert_zero(gb, srt[j]);
+                        outi[rr] = gb | lsp(row);
+                        if ((gb & lcm) != lcv) continue;
+                        outok[rr] = true;
+                        double ox = 0.0, oy = 0.0;
+                        for (uint32_t col = 0; col < dim; ++col) {
+                            const double2 mm = M[static_cast<uint64_t>(row) * dim + col];
+                            const V v = S[gb | lsp(col)];
+                            const T mx = static_cast<T>(mm.x), my = static_cast<T>(mm.y);
+                            ox += static_cast<double>(mx * v.x - my * v.y);
+                            oy += static_cast<double>(mx * v.y + my * v.x);
+                        }
+                        outv[rr] = mk<T>(static_cast<T>(ox), static_cast<T>(oy));
+                    }
+                    if (dot) {
+                        for (uint32_t rr = 0; rr < R; ++rr)
+                            if (outok[rr]) acc_dot(accr, acci, B[outi[rr]], outv[rr]);
+                    } else {
+                        __syncthreads();
+                        for (uint32_t rr = 0; rr < R; ++rr) {
+                            const uint32_t e = r * per_round + rr * kBlock + tid;
+                            if (e >= tsize) continue;
+                            if (outok[rr]) S[outi[rr]] = outv[rr];
+                            else if (zero_unctrl) S[outi[rr]] = zero;
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+        } else if (kind == TOP_DIAG || kind == TOP_PCPHASE) {
+            const int k = static_cast<int>(op.k);
+            uint32_t idx_out = 0;
+            for (int j = 0; j < k; ++j)
+                if (op.lpos[j] < 0) idx_out |= static_cast<uint32_t>((base >> op.gpos[j]) & 1ull) << (k - 1 - j);
+            const double2 *D = a.coef + op.coef_off;
+            // DIAG k=0 holds one entry; only PCPHASE has a second one
+            const V d0 = ld_coef<T>(D, 0);
+            const V d1 = (kind == TOP_PCPHASE) ? ld_coef<T>(D, 1) : d0;
+            for (uint32_t l = tid; l < tsize; l += kBlock) {
+                const bool ok = (l & lcm) == lcv;
+                uint32_t idx = idx_out;
+                for (int j = 0; j < k; ++j)
+                    if (op.lpos[j] >= 0) idx |= ((l >> op.lpos[j]) & 1u) << (k - 1 - j);
+                V d;
+                if (kind == TOP_PCPHASE) d = (idx < op.pc_dim) ? d0 : d1;
+                else d = (k == 0) ? d0 : ld_coef<T>(D, idx);
+                if (dot) {
+                    if (ok) acc_dot(accr, acci, B[l], cmul(d, A[l]));
+                } else {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if (!((smask >> s) & 1u)) continue;
+                        V *S = lds + s * tsize;
+                        if (ok) S[l] = cmul(d, S[l]);
+                        else if (zero_unctrl) S[l] = zero;
+                    }
+                }
+            }
+        } else if (kind == TOP_PARITY) {
+            const uint32_t pout = __popcll(base & op.omask) & 1u;
+            const V d0 = ld_coef<T>(a.coef, op.coef_off), d1 = ld_coef<T>(a.coef, op.coef_off + 1);
+            for (uint32_t l = tid; l < tsize; l += kBlock) {
+                const bool ok = (l & lcm) == lcv;
+                const uint32_t par = (__popc(l & op.lxmask) & 1u) ^ pout;
+                const V d = par ? d1 : d0;
+                if (dot) {
+                    if (ok) acc_dot(accr, acci, B[l], cmul(d, A[l]));
+                } else {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if (!((smask >> s) & 1u)) continue;
+                        V *S = lds + s * tsize;
+                        if (ok) S[l] = cmul(d, S[l]);
+                        else if (zero_unctrl) S[l] = zero;
+                    }
+                }
+            }
+        } else if (kind == TOP_PAULI) {
+            const uint32_t lx = op.lxmask, lz = op.lzmask;
+            const uint32_t sout = __popcll(base & op.omask) & 1u;
+            const V cp = ld_coef<T>(a.coef, op.coef_off);
+            const V cm = mk<T>(-cp.x, -cp.y);
+            if (dot) {
+                for (uint32_t l = tid; l < tsize; l += kBlock) {
+                    if ((l & lcm) != lcv) continue;
+                    const V f = ((__popc(l & lz) & 1u) ^ sout) ? cm : cp;
+                    acc_dot(accr, acci, B[l], cmul(f, A[l ^ lx]));
+                }
+            } else if (lx == 0) {
+                for (uint32_t l = tid; l < tsize; l += kBlock) {
+                    const bool ok = (l & lcm) == lcv;
+                    const V f = ((__popc(l & lz) & 1u) ^ sout) ? cm : cp;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if (!((smask >> s) & 1u)) continue;
+                        V *S = lds + s * tsize;
+                        if (ok) S[l] = cmul(f, S[l]);
+                        else if (zero_unctrl) S[l] = zero;
+                    }
+                }
+            } else {
+                const int hb = 31 - __clz(lx);
+                for (uint32_t p = tid; p < (tsize >> 1); p += kBlock) {
+                    const uint32_t i = insert_zero(p, hb), j = i ^ lx;
+                    const bool ok = (i & lcm) == lcv;
+                    const V fi = ((__popc(i & lz) & 1u) ^ sout) ? cm : cp;
+                    const V fj = ((__popc(j & lz) & 1u) ^ sout) ? cm : cp;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if (!((smask >> s) & 1u)) continue;
+                        V *S = lds + s * tsize;
+                        if (ok) {
+                            const V vi = S[i], vj = S[j];
+                            S[i] = cmul(fi, vj);
+                            S[j] = cmul(fj, vi);
+                        } else if (zero_unctrl) {
+                            S[i] = zero;
+                            S[j] = zero;
+                        }
+                    }
+                }
+            }
+        }
+
+        if (dot) block_reduce_store(accr, acci, red, dst);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (!((a.store_mask >> s) & 1u)) continue;
+        V *dstp = reinterpret_cast<V *>(a.sv[s]);
+        for (uint32_t it = 0; it < iters; ++it) {
+            const uint32_t l = tid + it * kBlock;
+            if (l < tsize) dstp[base | sp_tid | spread(it * kBlock)] = lds[s * tsize + l];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+// One block per slot; each thread keeps a Neumaier-compensated sum over a
+// strided subset, then a fixed-order tree.
+__global__ __launch_bounds__(kBlock) void reduce_slots_kernel(const double2 *partials, const SlotRange *ranges,
+                                                             double2 *out) {
+    __shared__ double2 red[kBlock / 64];
+    const SlotRange r = ranges[blockIdx.x];
+    double sr = 0.0, cr = 0.0, si = 0.0, ci = 0.0;
+    for (uint64_t i = threadIdx.x; i < r.count; i += kBlock) {
+        const double2 v = partials[r.off + i];
+        double t = sr + v.x;
+        cr += (fabs(sr) >= fabs(v.x)) ? (sr - t) + v.x : (v.x - t) + sr;
+        sr = t;
+        t = si + v.y;
+        ci += (fabs(si) >= fabs(v.y)) ? (si - t) + v.y : (v.y - t) + si;
+        si = t;
+    }
+    block_reduce_store(sr + cr, si + ci, red, &out[blockIdx.x]);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void pauli_sum_kernel(const typename VecT<T>::type *a, typename VecT<T>::type *out,
+                                                          const PauliTerm *terms, uint32_t nterms, uint64_t N,
+                                                          double2 *partials) {
+    using V = typename VecT<T>::type;
+    __shared__ double2 red[kBlock / 64];
+    double er = 0.0, ei = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride) {
+        double sr = 0.0, si = 0.0;
+        for (uint32_t t = 0; t < nterms; ++t) {
+            const PauliTerm pt = terms[t];
+            const V v = a[i ^ pt.x];
+            const double sg = (__popcll(i & pt.yz) & 1) ? -1.0 : 1.0;
+            const double cr = sg * pt.cre, ci = sg * pt.cim;
+            const double vx = v.x, vy = v.y;
+            sr += cr * vx - ci * vy;
+            si += cr * vy + ci * vx;
+        }
+        if (MODE == 1) {
+            V o;
+            o.x = static_cast<T>(sr);
+            o.y = static_cast<T>(si);
+            out[i] = o;
+        } else {
+            const V b = a[i];
+            const double bx = b.x, by = b.y;
+            er += bx * sr + by * si;
+            ei += bx * si - by * sr;
+        }
+    }
+    if (MODE == 0) block_reduce_store(er, ei, red, &partials[blockIdx.x]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void norm2_kernel(const typename VecT<T>::type *a, uint64_t N, double2 *partials) {
+    __shared__ double2 red[kBlock / 64];
+    double s = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride) {
+        const double x = a[i].x, y = a[i].y;
+        s += x * x + y * y;
+    }
+    block_reduce_store(s, 0.0, red, &partials[blockIdx.x]);
+}
+
+struct BitList {
+    int32_t b[64];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void probs_hist_kernel(const typename VecT<T>::type *a, uint64_t N, BitList bits,
+                                                           int nw, double *partials) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *hist = reinterpret_cast<double *>(smem_raw);
+    const uint32_t nb = 1u << nw;
+    for (uint32_t o = threadIdx.x; o < nb; o += kBlock) hist[o] = 0.0;
+    __syncthreads();
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const uint64_t nchunks = (N + stride - 1) / stride;
+    for (uint64_t ch = 0; ch < nchunks; ++ch) {
+        const uint64_t i = ch * stride + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+        double p = 0.0;
+        uint32_t o = 0;
+        if (i < N) {
+            const double x = a[i].x, y = a[i].y;
+            p = x * x + y * y;
+            for (int j = 0; j < nw; ++j) o |= static_cast<uint32_t>((i >> bits.b[j]) & 1ull) << (nw - 1 - j);
+        }
+        // wave-uniform bin (high wires): reduce across the wave, one atomic
+        const uint32_t o0 = __shfl(o, 0, 64);
+        if (__all(o == o0)) {
+            for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+            if ((threadIdx.x & 63u) == 0) atomicAdd(&hist[o0], p);
+        } else if (i < N) {
+            atomicAdd(&hist[o], p);
+        }
+    }
+    __syncthreads();
+    for (uint32_t o = threadIdx.x; o < nb; o += kBlock)
+        partials[static_cast<uint64_t>(blockIdx.x) * nb + o] = hist[o];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void probs_gather_kernel(const typename VecT<T>::type *a, BitList bits, int nw,
+                                                             BitList rest, int nrest, double *out) {
+    const uint64_t o = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (o >= (1ull << nw)) return;
+    uint64_t bi = 0;
+    for (int j = 0; j < nw; ++j) bi |= ((o >> (nw - 1 - j)) & 1ull) << bits.b[j];
+    double s = 0.0;
+    const uint64_t nr = 1ull << nrest;
+    for (uint64_t r = 0; r < nr; ++r) {
+        uint64_t idx = bi;
+        for (int j = 0; j < nrest; ++j) idx |= ((r >> j) & 1ull) << rest.b[j];
+        const double x = a[idx].x, y = a[idx].y;
+        s += x * x + y * y;
+    }
+    out[o] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const double *partials, uint32_t nrows, uint32_t ncols,
+                                                            double *out) {
+    const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
+    if (col >= ncols) return;
+    double s = 0.0, cmp = 0.0;
+    for (uint32_t r = 0; r < nrows; ++r) {
+        const double v = partials[static_cast<uint64_t>(r) * ncols + col];
+        const double t = s + v;
+        cmp += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+        s = t;
+    }
+    out[col] = s + cmp;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void fill_basis_kernel(typename VecT<T>::type *a, uint64_t N, uint64_t index) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride)
+        a[i] = mk<T>(i == index ? T(1) : T(0), T(0));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scale_kernel(typename VecT<T>::type *a, uint64_t N, double re, double im) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const typename VecT<T>::type f = mk<T>(static_cast<T>(re), static_cast<T>(im));
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride)
+        a[i] = cmul(f, a[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void collapse_kernel(typename VecT<T>::type *a, uint64_t N, int bit, int branch,
+                                                         double factor) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const T f = static_cast<T>(factor);
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride) {
+        if (static_cast<int>((i >> bit) & 1ull) != branch) a[i] = mk<T>(T(0), T(0));
+        else a[i] = mk<T>(a[i].x * f, a[i].y * f);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void set_subspace_kernel(typename VecT<T>::type *a, const typename VecT<T>::type *src,
+                                                             BitList bits, int nw) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= (1ull << nw)) return;
+    uint64_t idx = 0;
+    for (int j = 0; j < nw; ++j) idx |= ((i >> (nw - 1 - j)) & 1ull) << bits.b[j];
+    a[idx] = src[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void axpy_kernel(const typename VecT<T>::type *x, typename VecT<T>::type *y,
+                                                     uint64_t N, double re, double im) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const typename VecT<T>::type f = mk<T>(static_cast<T>(re), static_cast<T>(im));
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride) {
+        const typename VecT<T>::type v = cmul(f, x[i]);
+        y[i] = mk<T>(y[i].x + v.x, y[i].y + v.y);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void c128_to_c64_kernel(const double2 *s, float2 *d, uint64_t N) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride)
+        d[i] = make_float2(static_cast<float>(s[i].x), static_cast<float>(s[i].y));
+}
+__global__ __launch_bounds__(kBlock) void c64_to_c128_kernel(const float2 *s, double2 *d, uint64_t N) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride)
+        d[i] = make_double2(s[i].x, s[i].y);
+}
+
+uint32_t stream_blocks(uint64_t N) {
+    const uint64_t b = (N + kBlock - 1) / kBlock;
+    return static_cast<uint32_t>(b < 8192 ? (b ? b : 1) : 8192);
+}
+
+BitList to_bitlist(const int *bits, int nw) {
+    BitList b{};
+    for (int j = 0; j < nw && j < 64; ++j) b.b[j] = bits[j];
+    return b;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <typename T> void launch_tile_pass(const TilePassArgs &a, size_t lds_bytes, hipStream_t s) {
+    const uint64_t nt = a.ntiles;
+    const uint32_t gx = static_cast<uint32_t>(nt < 65536 ? nt : 65536);
+    const uint32_t gy = static_cast<uint32_t>((nt + gx - 1) / gx);
+    const dim3 grid(gx, gy);
+    switch (a.ns) {
+    case 1: hipLaunchKernelGGL((tile_pass_kernel<T, 1>), grid, dim3(kBlock), lds_bytes, s, a); break;
+    case 2: hipLaunchKernelGGL((tile_pass_kernel<T, 2>), grid, dim3(kBlock), lds_bytes, s, a); break;
+    case 3: hipLaunchKernelGGL((tile_pass_kernel<T, 3>), grid, dim3(kBlock), lds_bytes, s, a); break;
+    default: hipLaunchKernelGGL((tile_pass_kernel<T, 4>), grid, dim3(kBlock), lds_bytes, s, a); break;
+    }
+}
+template void launch_tile_pass<double>(const TilePassArgs &, size_t, hipStream_t);
+template void launch_tile_pass<float>(const TilePassArgs &, size_t, hipStream_t);
+
+void launch_reduce_slots(const double2 *partials, const SlotRange *ranges, uint32_t nslots, double2 *out,
+                         hipStream_t s) {
+    if (nslots == 0) return;
+    hipLaunchKernelGGL(reduce_slots_kernel, dim3(nslots), dim3(kBlock), 0, s, partials, ranges, out);
+}
+
+template <typename T>
+void launch_pauli_sum(const void *a, void *out, const PauliTerm *terms, uint32_t nterms, int n, int mode,
+                      double2 *partials, uint32_t nblocks, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    const uint64_t N = 1ull << n;
+    if (mode == 0)
+        hipLaunchKernelGGL((pauli_sum_kernel<T, 0>), dim3(nblocks), dim3(kBlock), 0, s, static_cast<const V *>(a),
+                           static_cast<V *>(out), terms, nterms, N, partials);
+    else
+        hipLaunchKernelGGL((pauli_sum_kernel<T, 1>), dim3(stream_blocks(N)), dim3(kBlock), 0, s,
+                           static_cast<const V *>(a), static_cast<V *>(out), terms, nterms, N, partials);
+}
+template void launch_pauli_sum<double>(const void *, void *, const PauliTerm *, uint32_t, int, int, double2 *, uint32_t,
+                                       hipStream_t);
+template void launch_pauli_sum<float>(const void *, void *, const PauliTerm *, uint32_t, int, int, double2 *, uint32_t,
+                                      hipStream_t);
+
+template <typename T> void launch_norm2(const void *a, int n, double2 *partials, uint32_t nblocks, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    hipLaunchKernelGGL(norm2_kernel<T>, dim3(nblocks), dim3(kBlock), 0, s, static_cast<const V *>(a), 1ull << n,
+                       partials);
+}
+template void launch_norm2<double>(const void *, int, double2 *, uint32_t, hipStream_t);
+template void launch_norm2<float>(const void *, int, double2 *, uint32_t, hipStream_t);
+
+template <typename T>
+void launch_probs_hist(const void *a, int n, const int *bits, int nw, double *partials, uint32_t nblocks,
+                       hipStream_t s) {
+    using V = typename VecT<T>::type;
+    const size_t lds = sizeof(double) << nw;
+    hipLaunchKernelGGL(probs_hist_kernel<T>, dim3(nblocks), dim3(kBlock), lds, s, static_cast<const V *>(a), 1ull << n,
+                       to_bitlist(bits, nw), nw, partials);
+}
+template void launch_probs_hist<double>(const void *, int, const int *, int, double *, uint32_t, hipStream_t);
+template void launch_probs_hist<float>(const void *, int, const int *, int, double *, uint32_t, hipStream_t);
+
+template <typename T>
+void launch_probs_gather(const void *a, int n, const int *bits, int nw, double *out, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    int rest[64];
+    int nr = 0;
+    for (int b = 0; b < n; ++b) {
+        bool in = false;
+        for (int j = 0; j < nw; ++j) in = in || bits[j] == b;
+        if (!in) rest[nr++] = b;
+    }
+    const uint64_t no = 1ull << nw;
+    hipLaunchKernelGGL(probs_gather_kernel<T>, dim3(static_cast<uint32_t>((no + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, static_cast<const V *>(a), to_bitlist(bits, nw), nw, to_bitlist(rest, nr), nr, out);
+}
+template void launch_probs_gather<double>(const void *, int, const int *, int, double *, hipStream_t);
+template void launch_probs_gather<float>(const void *, int, const int *, int, double *, hipStream_t);
+
+void launch_reduce_rows(const double *partials, uint32_t nrows, uint32_t ncols, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3((ncols + kBlock - 1) / kBlock), dim3(kBlock), 0, s, partials, nrows,
+                       ncols, out);
+}
+
+template <typename T> void launch_fill_basis(void *a, int n, uint64_t index, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    const uint64_t N = 1ull << n;
+    hipLaunchKernelGGL(fill_basis_kernel<T>, dim3(stream_blocks(N)), dim3(kBlock), 0, s, static_cast<V *>(a), N, index);
+}
+template void launch_fill_basis<double>(void *, int, uint64_t, hipStream_t);
+template void launch_fill_basis<float>(void *, int, uint64_t, hipStream_t);
+
+template <typename T> void launch_scale(void *a, int n, double re, double im, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    const uint64_t N = 1ull << n;
+    hipLaunchKernelGGL(scale_kernel<T>, dim3(stream_blocks(N)), dim3(kBlock), 0, s, static_cast<V *>(a), N, re, im);
+}
+template void launch_scale<double>(void *, int, double, double, hipStream_t);
+template void launch_scale<float>(void *, int, double, double, hipStream_t);
+
+template <typename T> void launch_collapse(void *a, int n, int bit, int branch, double factor, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    const uint64_t N = 1ull << n;
+    hipLaunchKernelGGL(collapse_kernel<T>, dim3(stream_blocks(N)), dim3(kBlock), 0, s, static_cast<V *>(a), N, bit,
+                       branch, factor);
+}
+template void launch_collapse<double>(void *, int, int, int, double, hipStream_t);
+template void launch_collapse<float>(void *, int, int, int, double, hipStream_t);
+
+template <typename T>
+void launch_set_subspace(void *a, int n, const void *src, const int *bits, int nw, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    (void)n;
+    const uint64_t no = 1ull << nw;
+    hipLaunchKernelGGL(set_subspace_kernel<T>, dim3(static_cast<uint32_t>((no + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, static_cast<V *>(a), static_cast<const V *>(src), to_bitlist(bits, nw), nw);
+}
+template void launch_set_subspace<double>(void *, int, const void *, const int *, int, hipStream_t);
+template void launch_set_subspace<float>(void *, int, const void *, const int *, int, hipStream_t);
+
+template <typename T> void launch_axpy(const void *x, void *y, double re, double im, int n, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    const uint64_t N = 1ull << n;
+    hipLaunchKernelGGL(axpy_kernel<T>, dim3(stream_blocks(N)), dim3(kBlock), 0, s, static_cast<const V *>(x),
+                       static_cast<V *>(y), N, re, im);
+}
+template void launch_axpy<double>(const void *, void *, double, double, int, hipStream_t);
+template void launch_axpy<float>(const void *, void *, double, double, int, hipStream_t);
+
+void launch_c128_to_c64(const void *src, void *dst, uint64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(c128_to_c64_kernel, dim3(stream_blocks(count)), dim3(kBlock), 0, s,
+                       static_cast<const double2 *>(src), static_cast<float2 *>(dst), count);
+}
+void launch_c64_to_c128(const void *src, void *dst, uint64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(c64_to_c128_kernel, dim3(stream_blocks(count)), dim3(kBlock), 0, s,
+                       static_cast<const float2 *>(src), static_cast<double2 *>(dst), count);
+}
+
+} // namespace lqamd

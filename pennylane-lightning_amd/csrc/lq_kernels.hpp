@@ -1,0 +1,81 @@
+// Launch interface of the HIP kernels (lq_kernels.hip). Host code calls these
+// launchers; every launch goes on the caller's stream and returns without
+// synchronising.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lq_tileop.hpp"
+
+namespace lqamd {
+
+struct TilePassArgs {
+    void *sv[kMaxStates];     // state base pointers (complex<T>)
+    const TileOp *ops;        // device program for this pass
+    const double2 *coef;      // device coefficient buffer (complex double)
+    double2 *partials;        // dot partial sums: [slot][tile], slots local to the pass
+    uint32_t nops;
+    uint32_t load_mask;       // states loaded into LDS
+    uint32_t store_mask;      // states written back
+    uint32_t ns;              // number of state slots in LDS
+    uint64_t ntiles;
+    TileGeom g;
+};
+
+struct PauliTerm {
+    uint64_t x;   // flipped bits
+    uint64_t yz;  // bits contributing (-1)^bit (Y and Z factors)
+    double cre;   // coefficient times (-i)^ny, real part
+    double cim;
+};
+
+struct SlotRange {
+    uint64_t off;   // offset (complex entries) of the slot's partials
+    uint64_t count; // number of partials
+};
+
+template <typename T> void launch_tile_pass(const TilePassArgs &a, size_t lds_bytes, hipStream_t s);
+
+// out[slot] = sum of partials (Neumaier-compensated, fixed order)
+void launch_reduce_slots(const double2 *partials, const SlotRange *ranges, uint32_t nslots,
+                         double2 *out, hipStream_t s);
+
+// mode 0: partials[block] = sum_t c_t sum_i conj(a[i]) P_t a[i]   (expectation)
+// mode 1: out[i] = sum_t c_t (P_t a)[i]                             (apply)
+template <typename T>
+void launch_pauli_sum(const void *a, void *out, const PauliTerm *terms, uint32_t nterms, int n,
+                      int mode, double2 *partials, uint32_t nblocks, hipStream_t s);
+
+// sum_i |a_i|^2 into per-block partials
+template <typename T>
+void launch_norm2(const void *a, int n, double2 *partials, uint32_t nblocks, hipStream_t s);
+
+// marginal probabilities: out[o] over the wires given as physical bits
+// (bits[0] is the output MSB). Small nw: per-block LDS histograms into
+// partials[block][o]; large nw: one output per thread.
+template <typename T>
+void launch_probs_hist(const void *a, int n, const int *bits, int nw, double *partials,
+                       uint32_t nblocks, hipStream_t s);
+template <typename T>
+void launch_probs_gather(const void *a, int n, const int *bits, int nw, double *out, hipStream_t s);
+void launch_reduce_rows(const double *partials, uint32_t nrows, uint32_t ncols, double *out,
+                        hipStream_t s);
+
+template <typename T> void launch_fill_basis(void *a, int n, uint64_t index, hipStream_t s);
+template <typename T> void launch_scale(void *a, int n, double re, double im, hipStream_t s);
+// zero amplitudes whose bit `bit` != branch, then scale the rest by `factor`
+template <typename T>
+void launch_collapse(void *a, int n, int bit, int branch, double factor, hipStream_t s);
+// a[dst_index(i)] = src[i] for i < 2^nw with all other bits zero; the rest is
+// left as it is (lq/StateVectorLQubit.hpp:1005-1036)
+template <typename T>
+void launch_set_subspace(void *a, int n, const void *src, const int *bits, int nw, hipStream_t s);
+// y += (re + i im) * x
+template <typename T> void launch_axpy(const void *x, void *y, double re, double im, int n, hipStream_t s);
+// convert between complex128 and complex64 device buffers
+void launch_c128_to_c64(const void *src, void *dst, uint64_t count, hipStream_t s);
+void launch_c64_to_c128(const void *src, void *dst, uint64_t count, hipStream_t s);
+
+} // namespace lqamd

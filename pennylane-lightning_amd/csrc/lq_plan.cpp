@@ -1,0 +1,453 @@
+// Lowering and pass planning (host).
+//
+// Gate semantics follow the reference: controls select the subspace
+// ctrl == value (GateImplementationsLM.hpp:407-440); generators zero the
+// non-control subspace (:2181-2228); MultiRZ is a parity phase (:1988-2010);
+// PCPhase is a threshold phase on the local index (:2113-2162).
+//
+// Planning: ops form a DAG in which two ops commute when every bit they share
+// is acted on diagonally by both (controls, diagonal targets), or when both
+// are read-only dots. A pass greedily takes ready ops whose non-diagonal bits
+// fit in its tile (at most kt_max bits, the low bits always included), up to
+// a compute budget; diagonal ops fit any pass.
+#include "lq_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <set>
+
+namespace lqamd {
+
+namespace {
+inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+bool all_zero(const std::vector<cd> &m) {
+    for (const auto &x : m)
+        if (x != cd(0.0)) return false;
+    return true;
+}
+bool is_identity(const std::vector<cd> &m, int d) {
+    for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c)
+            if (m[static_cast<size_t>(r) * d + c] != (r == c ? cd(1.0) : cd(0.0))) return false;
+    return true;
+}
+bool is_diagonal(const std::vector<cd> &m, int d) {
+    for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c)
+            if (r != c && m[static_cast<size_t>(r) * d + c] != cd(0.0)) return false;
+    return true;
+}
+} // namespace
+
+uint64_t LOp::target_mask() const {
+    uint64_t m = 0;
+    for (int j = 0; j < k; ++j) m |= 1ull << bits[j];
+    return m;
+}
+
+uint64_t LOp::nmask() const {
+    switch (kind) {
+    case TOP_DENSE1:
+    case TOP_DENSE2:
+    case TOP_DENSEK: return target_mask();
+    case TOP_PAULI: return xmask;
+    default: return 0;
+    }
+}
+
+uint64_t LOp::dmask() const {
+    uint64_t m = cmask;
+    switch (kind) {
+    case TOP_DIAG:
+    case TOP_PCPHASE: m |= target_mask(); break;
+    case TOP_PARITY: m |= pmask; break;
+    case TOP_PAULI: m |= yzmask & ~xmask; break;
+    default: break;
+    }
+    return m;
+}
+
+double LOp::cost() const {
+    double c = 0.5;
+    switch (kind) {
+    case TOP_DENSE1: c = 1.0; break;
+    case TOP_DENSE2: c = 2.5; break;
+    case TOP_DENSEK: c = static_cast<double>(1 << k) / 2.0; break;
+    case TOP_DIAG: c = 0.5; break;
+    case TOP_PARITY: c = 0.4; break;
+    case TOP_PCPHASE: c = 0.5; break;
+    case TOP_PAULI: c = 0.6; break;
+    default: break;
+    }
+    if (is_dot()) return c + 0.5;
+    return c * std::max(1, popc(smask));
+}
+
+LOp lower_matrix(std::vector<cd> mat, std::vector<int> tbits, uint64_t cmask, uint64_t cval, bool gen) {
+    int k = static_cast<int>(tbits.size());
+    LQ_ABORT_IF(mat.size() != (static_cast<size_t>(1) << (2 * k)), "The size of matrix does not match with the given number of wires");
+    bool changed = true;
+    while (changed && k > 0) {
+        changed = false;
+        for (int j = 0; j < k && !changed; ++j) {
+            const int d = 1 << k, h = d / 2, lb = k - 1 - j;
+            bool offzero = true;
+            for (int r = 0; r < d && offzero; ++r)
+                for (int c = 0; c < d; ++c)
+                    if ((((r >> lb) ^ (c >> lb)) & 1) && mat[static_cast<size_t>(r) * d + c] != cd(0.0)) {
+                        offzero = false;
+                        break;
+                    }
+            if (!offzero) continue;
+            std::vector<cd> A(static_cast<size_t>(h) * h), B(static_cast<size_t>(h) * h);
+            auto ins = [lb](int x, int v) { return ((x >> lb) << (lb + 1)) | (v << lb) | (x & ((1 << lb) - 1)); };
+            for (int r = 0; r < h; ++r)
+                for (int c = 0; c < h; ++c) {
+                    A[static_cast<size_t>(r) * h + c] = mat[static_cast<size_t>(ins(r, 0)) * d + ins(c, 0)];
+                    B[static_cast<size_t>(r) * h + c] = mat[static_cast<size_t>(ins(r, 1)) * d + ins(c, 1)];
+                }
+            int ctrl = -1;
+            std::vector<cd> *keep = nullptr;
+            if (A == B) {
+                keep = &A;
+            } else if (!gen && is_identity(A, h)) {
+                keep = &B;
+                ctrl = 1;
+            } else if (!gen && is_identity(B, h)) {
+                keep = &A;
+                ctrl = 0;
+            } else if (gen && all_zero(A)) {
+                keep = &B;
+                ctrl = 1;
+            } else if (gen && all_zero(B)) {
+                keep = &A;
+                ctrl = 0;
+            }
+            if (!keep) continue;
+            if (ctrl >= 0) {
+                cmask |= 1ull << tbits[j];
+                if (ctrl) cval |= 1ull << tbits[j];
+            }
+            mat = *keep;
+            tbits.erase(tbits.begin() + j);
+            --k;
+            changed = true;
+        }
+    }
+    LOp op;
+    op.cmask = cmask;
+    op.cval = cval;
+    op.k = k;
+    for (int j = 0; j < k; ++j) op.bits[j] = tbits[j];
+    const int d = 1 << k;
+    if (k == 0 || is_diagonal(mat, d)) {
+        LQ_ABORT_IF(k > kMaxTileOpTargets, "diagonal operator on too many wires");
+        op.kind = TOP_DIAG;
+        op.coef.resize(d);
+        for (int i = 0; i < d; ++i) op.coef[i] = mat[static_cast<size_t>(i) * d + i];
+        return op;
+    }
+    LQ_ABORT_IF(k > 10, "dense operators are supported on at most 10 wires");
+    op.kind = k == 1 ? TOP_DENSE1 : k == 2 ? TOP_DENSE2 : TOP_DENSEK;
+    op.coef = std::move(mat);
+    return op;
+}
+
+namespace {
+void controls_to_mask(int n, const uint32_t *ctrls, const uint8_t *cvals, int nc, uint64_t *cm, uint64_t *cv) {
+    *cm = 0;
+    *cv = 0;
+    for (int i = 0; i < nc; ++i) {
+        const int b = phys_bit(n, ctrls[i]);
+        *cm |= 1ull << b;
+        if (cvals == nullptr || cvals[i]) *cv |= 1ull << b;
+    }
+}
+} // namespace
+
+LOp lower_gate(int n, GateId g, const uint32_t *wires, int nw, const uint32_t *ctrls, const uint8_t *cvals, int nc,
+               bool inverse, const double *params) {
+    uint64_t cm, cv;
+    controls_to_mask(n, ctrls, cvals, nc, &cm, &cv);
+    std::vector<int> tb(nw);
+    for (int j = 0; j < nw; ++j) tb[j] = phys_bit(n, wires[j]);
+    const cd I1(0.0, 1.0);
+    if (g == GateId::MultiRZ) {
+        LOp op;
+        op.kind = TOP_PARITY;
+        op.cmask = cm;
+        op.cval = cv;
+        for (int b : tb) op.pmask |= 1ull << b;
+        const cd first = std::exp(-I1 * (params[0] / 2)), second = std::exp(I1 * (params[0] / 2));
+        op.coef = {inverse ? std::conj(first) : first, inverse ? std::conj(second) : second};
+        return op;
+    }
+    if (g == GateId::PCPhase) {
+        LQ_ABORT_IF(nw > kMaxTileOpTargets, "PCPhase is supported on at most 12 wires");
+        const double dim = std::round(params[1]);
+        LQ_ABORT_IF(dim < 0 || dim > std::ldexp(1.0, n),
+                    "The dimension of the PCPhase gate must be a positive integer and less than or equal to statevector size.");
+        LOp op;
+        op.kind = TOP_PCPHASE;
+        op.cmask = cm;
+        op.cval = cv;
+        op.k = nw;
+        for (int j = 0; j < nw; ++j) op.bits[j] = tb[j];
+        op.pc_dim = static_cast<uint64_t>(dim);
+        const double ph = inverse ? -params[0] : params[0];
+        op.coef = {std::exp(I1 * ph), std::exp(-I1 * ph)};
+        return op;
+    }
+    if (g == GateId::GlobalPhase) {
+        LOp op;
+        op.kind = TOP_DIAG;
+        op.k = 0;
+        op.cmask = cm;
+        op.cval = cv;
+        op.coef = {std::exp(cd(0.0, inverse ? params[0] : -params[0]))};
+        return op;
+    }
+    return lower_matrix(gate_matrix(g, params, inverse, nw), tb, cm, cv, false);
+}
+
+LOp lower_generator(int n, GateId g, const uint32_t *wires, int nw, const uint32_t *ctrls, const uint8_t *cvals,
+                    int nc, double *scale) {
+    uint64_t cm, cv;
+    controls_to_mask(n, ctrls, cvals, nc, &cm, &cv);
+    std::vector<int> tb(nw);
+    for (int j = 0; j < nw; ++j) tb[j] = phys_bit(n, wires[j]);
+    LOp op;
+    if (g == GateId::MultiRZ) {
+        op.kind = TOP_PARITY;
+        op.cmask = cm;
+        op.cval = cv;
+        for (int b : tb) op.pmask |= 1ull << b;
+        op.coef = {1.0, -1.0};
+        *scale = -0.5;
+    } else if (g == GateId::GlobalPhase) {
+        op.kind = TOP_DIAG;
+        op.k = 0;
+        op.cmask = cm;
+        op.cval = cv;
+        op.coef = {1.0};
+        *scale = -1.0;
+    } else {
+        Generator gen = gate_generator(g, nw);
+        op = lower_matrix(gen.g, tb, cm, cv, true);
+        *scale = gen.scale;
+    }
+    op.flags |= TF_ZERO_UNCTRL;
+    return op;
+}
+
+namespace {
+uint64_t touch_mask(const LOp &o) { return o.target_mask() | o.cmask | o.pmask | o.xmask | o.yzmask; }
+
+bool single_bit_1q(const LOp &o) {
+    return !o.is_dot() && !(o.flags & TF_ZERO_UNCTRL) && o.k == 1 && (o.kind == TOP_DENSE1 || o.kind == TOP_DIAG);
+}
+
+std::vector<cd> as_2x2(const LOp &o) {
+    if (o.kind == TOP_DENSE1) return o.coef;
+    return {o.coef[0], 0.0, 0.0, o.coef[1]};
+}
+} // namespace
+
+bool try_merge_1q(std::vector<LOp> &queue, const LOp &op) {
+    if (!single_bit_1q(op)) return false;
+    const uint64_t mine = touch_mask(op);
+    const int lim = std::max(0, static_cast<int>(queue.size()) - 256);
+    for (int i = static_cast<int>(queue.size()) - 1; i >= lim; --i) {
+        LOp &q = queue[i];
+        if (!(touch_mask(q) & mine)) continue;
+        if (!single_bit_1q(q) || q.bits[0] != op.bits[0] || q.cmask != op.cmask || q.cval != op.cval ||
+            q.smask != op.smask)
+            return false;
+        const auto a = as_2x2(op), b = as_2x2(q);
+        std::vector<cd> m = {a[0] * b[0] + a[1] * b[2], a[0] * b[1] + a[1] * b[3], a[2] * b[0] + a[3] * b[2],
+                             a[2] * b[1] + a[3] * b[3]};
+        if (m[1] == cd(0.0) && m[2] == cd(0.0)) {
+            q.kind = TOP_DIAG;
+            q.coef = {m[0], m[3]};
+        } else {
+            q.kind = TOP_DENSE1;
+            q.coef = m;
+        }
+        return true;
+    }
+    return false;
+}
+
+TileGeom make_geom(int n, const std::vector<int> &tbits) {
+    TileGeom g{};
+    g.n = n;
+    g.kt = static_cast<int>(tbits.size());
+    g.nout = n - g.kt;
+    LQ_ABORT_IF(g.kt > 16 || g.nout > kMaxOutsideBits, "tile geometry out of range");
+    uint64_t tm = 0;
+    for (int j = 0; j < g.kt; ++j) {
+        g.tb[j] = tbits[j];
+        tm |= 1ull << tbits[j];
+    }
+    g.c = 0;
+    while (g.c < g.kt && g.tb[g.c] == g.c) ++g.c;
+    int o = 0;
+    for (int b = 0; b < n; ++b)
+        if (!((tm >> b) & 1ull)) g.ob[o++] = b;
+    return g;
+}
+
+std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig &cfg) {
+    const int N = static_cast<int>(ops.size());
+    const int n = cfg.n;
+    std::vector<std::vector<int>> succ(N);
+    std::vector<int> indeg(N, 0), mark(N, -1);
+    std::vector<uint64_t> nm(N), dm(N);
+    struct BitDeps {
+        int lastN = -1;
+        std::vector<int> D, RN, RD;
+    };
+    std::vector<BitDeps> bd(static_cast<size_t>(n));
+    for (int j = 0; j < N; ++j) {
+        nm[j] = ops[j].nmask();
+        dm[j] = ops[j].dmask() & ~nm[j];
+        LQ_ABORT_IF(((nm[j] | dm[j]) >> n) != 0, "operation acts on a wire outside the state");
+        const bool dot = ops[j].is_dot();
+        auto dep = [&](int i) {
+            if (i >= 0 && mark[i] != j) {
+                mark[i] = j;
+                succ[i].push_back(j);
+                ++indeg[j];
+            }
+        };
+        for (int b = 0; b < n; ++b) {
+            const uint64_t bit = 1ull << b;
+            BitDeps &s = bd[b];
+            if (nm[j] & bit) {
+                dep(s.lastN);
+                if (!dot) {
+                    for (int i : s.D) dep(i);
+                    for (int i : s.RN) dep(i);
+                    for (int i : s.RD) dep(i);
+                    s.lastN = j;
+                    s.D.clear();
+                    s.RN.clear();
+                    s.RD.clear();
+                } else {
+                    for (int i : s.D) dep(i);
+                    s.RN.push_back(j);
+                }
+            } else if (dm[j] & bit) {
+                dep(s.lastN);
+                if (!dot) {
+                    for (int i : s.RN) dep(i);
+                    s.D.push_back(j);
+                } else {
+                    s.RD.push_back(j);
+                }
+            }
+        }
+    }
+
+    std::set<int> ready;
+    for (int j = 0; j < N; ++j)
+        if (indeg[j] == 0) ready.insert(j);
+
+    const int ktmax = std::min(cfg.kt_max, n);
+    const int low = std::max(0, std::min(cfg.low_bits, ktmax));
+    std::vector<PassPlan> passes;
+    int remaining = N;
+    while (remaining > 0) {
+        std::vector<int> sel;
+        uint64_t T = 0;
+        for (int attempt = 0; attempt < 2 && sel.empty(); ++attempt) {
+            T = attempt == 0 ? ((low >= 64) ? ~0ull : ((1ull << low) - 1)) : 0ull;
+            double cost = 0.0;
+            while (true) {
+                int pick = -1;
+                for (int i : ready)
+                    if ((nm[i] & ~T) == 0) {
+                        pick = i;
+                        break;
+                    }
+                if (pick < 0)
+                    for (int i : ready)
+                        if (popc(T | nm[i]) <= ktmax) {
+                            pick = i;
+                            break;
+                        }
+                if (pick < 0) break;
+                if (!sel.empty() && cost + ops[pick].cost() > cfg.budget) break;
+                T |= nm[pick];
+                cost += ops[pick].cost();
+                sel.push_back(pick);
+                ready.erase(pick);
+                --remaining;
+                for (int s : succ[pick])
+                    if (--indeg[s] == 0) ready.insert(s);
+            }
+        }
+        LQ_ABORT_IF(sel.empty(), "operation needs more tile bits than fit in LDS");
+        // pad the tile with the lowest free bits
+        for (int b = 0; b < n && popc(T) < ktmax; ++b) T |= 1ull << b;
+
+        PassPlan p;
+        int pos[64];
+        for (int b = 0; b < 64; ++b) pos[b] = -1;
+        for (int b = 0; b < n; ++b)
+            if ((T >> b) & 1ull) {
+                pos[b] = static_cast<int>(p.tbits.size());
+                p.tbits.push_back(b);
+            }
+        auto local = [&](uint64_t m) {
+            uint32_t r = 0;
+            for (int b = 0; b < n; ++b)
+                if (((m >> b) & 1ull) && pos[b] >= 0) r |= 1u << pos[b];
+            return r;
+        };
+        for (int idx : sel) {
+            const LOp &o = ops[idx];
+            TileOp t{};
+            t.kind = o.kind;
+            t.flags = o.flags;
+            t.k = static_cast<uint32_t>(o.k);
+            t.smask = o.smask;
+            t.sa = o.sa;
+            t.sb = o.sb;
+            t.lcmask = local(o.cmask);
+            t.lcval = local(o.cval & o.cmask);
+            t.ocmask = o.cmask & ~T;
+            t.ocval = o.cval & o.cmask & ~T;
+            for (int j = 0; j < kMaxTileOpTargets; ++j) {
+                t.lpos[j] = j < o.k ? pos[o.bits[j]] : -1;
+                t.gpos[j] = j < o.k ? o.bits[j] : 0;
+            }
+            if (o.kind == TOP_PARITY) {
+                t.lxmask = local(o.pmask);
+                t.omask = o.pmask & ~T;
+            } else if (o.kind == TOP_PAULI) {
+                t.lxmask = local(o.xmask);
+                t.lzmask = local(o.yzmask);
+                t.omask = o.yzmask & ~T;
+                t.ny = static_cast<uint32_t>(o.ny);
+            }
+            t.pc_dim = o.pc_dim;
+            t.coef_off = p.coef.size();
+            p.coef.insert(p.coef.end(), o.coef.begin(), o.coef.end());
+            if (o.is_dot()) {
+                t.slot = static_cast<uint32_t>(p.slot_ids.size());
+                p.slot_ids.push_back(o.slot);
+                p.load_mask |= (1u << o.sa) | (1u << o.sb);
+            } else {
+                p.load_mask |= o.smask;
+                p.store_mask |= o.smask;
+            }
+            p.ops.push_back(t);
+        }
+        passes.push_back(std::move(p));
+    }
+    return passes;
+}
+
+} // namespace lqamd

@@ -1,0 +1,81 @@
+// Lowering of gates/generators/observables to physical-bit ops, and the pass
+// planner that groups them into LDS tile programs.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "lq_common.hpp"
+#include "lq_gates.hpp"
+#include "lq_tileop.hpp"
+
+namespace lqamd {
+
+// An op on physical bits (bit b = wire n-1-b), before tiling.
+struct LOp {
+    uint32_t kind = TOP_DIAG;
+    uint32_t flags = 0;
+    int k = 0;
+    int bits[kMaxTileOpTargets] = {}; // targets; bits[0] is the local MSB
+    uint64_t cmask = 0, cval = 0;     // controls
+    uint64_t xmask = 0, yzmask = 0;   // PAULI
+    int ny = 0;                       // PAULI
+    uint64_t pmask = 0;               // PARITY
+    uint64_t pc_dim = 0;              // PCPHASE
+    std::vector<cd> coef;
+    uint32_t smask = 1;               // gate ops: states it applies to
+    uint32_t sa = 0, sb = 0;          // dots: ket / bra
+    int32_t slot = -1;                // dots: caller's slot id
+
+    bool is_dot() const { return (flags & TF_DOT) != 0; }
+    uint64_t target_mask() const;
+    // bits that must be tile bits (non-diagonal action)
+    uint64_t nmask() const;
+    // bits acted on diagonally (controls, diagonal targets, Z parts)
+    uint64_t dmask() const;
+    double cost() const;
+};
+
+inline int phys_bit(int n, uint32_t wire) { return n - 1 - static_cast<int>(wire); }
+
+// Dense matrix (row-major 2^k x 2^k, k = tbits.size(), tbits[0] local MSB) on
+// physical bits with controls. Peels target bits on which the matrix is
+// block diagonal with an identity block (unitary mode) or a zero block
+// (generator mode) into controls, drops bits it acts on trivially, and emits
+// DIAG when the rest is diagonal.
+LOp lower_matrix(std::vector<cd> mat, std::vector<int> tbits, uint64_t cmask, uint64_t cval, bool generator_mode);
+
+// Named gate (lq/StateVectorLQubit.hpp:378-423 semantics)
+LOp lower_gate(int n, GateId g, const uint32_t *wires, int nw, const uint32_t *ctrls, const uint8_t *cvals, int nc,
+               bool inverse, const double *params);
+// Generator as an op on the state (non-control subspace zeroed) or as a dot;
+// returns the scale through *scale.
+LOp lower_generator(int n, GateId g, const uint32_t *wires, int nw, const uint32_t *ctrls, const uint8_t *cvals,
+                    int nc, double *scale);
+
+// Peephole: fold `op` into the last queued op when both act on exactly the
+// same single target bit with the same controls and nothing in between
+// touches that bit. Returns true if merged.
+bool try_merge_1q(std::vector<LOp> &queue, const LOp &op);
+
+struct PassPlan {
+    std::vector<int> tbits;        // tile bits, ascending
+    std::vector<TileOp> ops;
+    std::vector<cd> coef;
+    uint32_t load_mask = 0, store_mask = 0;
+    std::vector<int32_t> slot_ids; // caller slot id of each local dot slot
+};
+
+struct PlanConfig {
+    int n = 0;
+    int ns = 1;          // states resident per pass
+    int kt_max = 12;     // max tile bits
+    int low_bits = 4;    // contiguous low bits always in the tile
+    double budget = 24;  // max op cost per pass
+};
+
+std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig &cfg);
+
+TileGeom make_geom(int n, const std::vector<int> &tbits);
+
+} // namespace lqamd

@@ -1,0 +1,514 @@
+"""Drop-in mirror of ``pennylane_lightning.lightning_qubit_ops`` on MI355X.
+
+Class and method names, argument meaning and error behaviour follow the
+reference bindings (pennylane_lightning/core/bindings/Bindings.hpp,
+pennylane_lightning/core/simulators/lightning_qubit/bindings/LQubitBindings.hpp):
+
+* ``StateVectorC64`` / ``StateVectorC128`` (Bindings.hpp:870-922,
+  LQubitBindings.hpp:142-243): one method per gate name with the signatures
+  ``Gate(wires, inverse, params)`` and
+  ``Gate(controlled_wires, controlled_values, wires, inverse=False, params=[])``,
+  ``applyMatrix``, ``applyControlledMatrix``, ``applyPauliRot``,
+  ``setBasisState``, ``setStateVector``, ``resetStateVector``,
+  ``updateData``, ``getState``, ``normalize``, ``collapse``, ``kernel_map``.
+* ``MeasurementsC64`` / ``MeasurementsC128`` (Bindings.hpp:650-688,
+  LQubitBindings.hpp:268-350): ``probs``, ``expval`` (observable, name or
+  matrix form).
+* ``observables.*`` (Bindings.hpp:467-591) and ``algorithms.*``
+  (Bindings.hpp:698-851): ``OpsStructC*``, ``create_ops_listC*``,
+  ``AdjointJacobianC*``.
+
+Failures raise ``LightningException`` (a ``RuntimeError``) with the
+reference's message format. The state lives in HBM; gate calls are queued
+and run as fused passes when a result is read.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import types
+from typing import Sequence
+
+import numpy as np
+
+from . import _capi
+from ._capi import LightningException, check, f64, lib, u8, u32
+
+GATE_NAMES = [
+    "Identity", "PauliX", "PauliY", "PauliZ", "Hadamard", "S", "SX", "T", "PhaseShift", "RX", "RY", "RZ",
+    "Rot", "CNOT", "CY", "CZ", "SWAP", "IsingXX", "IsingXY", "IsingYY", "IsingZZ", "ControlledPhaseShift",
+    "CRX", "CRY", "CRZ", "CRot", "SingleExcitation", "SingleExcitationMinus", "SingleExcitationPlus",
+    "Toffoli", "CSWAP", "DoubleExcitation", "DoubleExcitationMinus", "DoubleExcitationPlus", "MultiRZ",
+    "GlobalPhase", "PSWAP", "PCPhase",
+]
+
+_DTYPES = {64: (0, np.complex64, np.float32), 128: (1, np.complex128, np.float64)}
+
+
+def backend_info():
+    return {"NAME": "lightning.qubit (MI355X)", "ENGINE": lib.lq_backend_info().decode()}
+
+
+def compile_info():
+    return {"cpu.arch": "x86_64", "device.arch": "gfx950", "compiler.name": "hipcc"}
+
+
+def runtime_info():
+    n = C.c_int()
+    check(lib.lq_device_count(C.byref(n)))
+    return {"devices": n.value}
+
+
+def _is_bool(x) -> bool:
+    return isinstance(x, (bool, np.bool_))
+
+
+class _StateVectorBase:
+    _bits = 128
+
+    def __init__(self, num_qubits: int):
+        code, self._cdtype, self._rdtype = _DTYPES[self._bits]
+        h = C.c_void_p()
+        check(lib.lq_sv_create(int(num_qubits), code, C.byref(h)))
+        self._h = h
+        self._n = int(num_qubits)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.lq_sv_destroy(h)
+            self._h = None
+
+    # -- size / state preparation -------------------------------------------
+    def __len__(self):
+        return 1 << self._n
+
+    def size(self):
+        return 1 << self._n
+
+    @property
+    def num_qubits(self):
+        return self._n
+
+    def resetStateVector(self):
+        check(lib.lq_sv_reset(self._h))
+
+    def setBasisState(self, state=None, wires=None, async_=False, **kw):
+        state = [] if state is None else list(state)
+        wires = list(range(len(state))) if wires is None else list(wires)
+        b, bp, _ = u32(state)
+        w, wp, nw = u32(wires)
+        check(lib.lq_sv_set_basis_state(self._h, bp, wp, nw))
+
+    def setStateVector(self, state, wires, async_=False, **kw):
+        s = np.ascontiguousarray(np.asarray(state, dtype=self._cdtype).reshape(-1))
+        w, wp, nw = u32(wires)
+        if s.size != (1 << nw):
+            raise LightningException("Inconsistent state and wires dimensions.")
+        check(lib.lq_sv_set_state_vector(self._h, s.ctypes.data_as(C.c_void_p), wp, nw))
+
+    def updateData(self, data):
+        a = np.asarray(data)
+        if a.ndim != 1:
+            raise ValueError("Array must be 1-dimensional")
+        a = np.ascontiguousarray(a, dtype=self._cdtype)
+        check(lib.lq_sv_update_data(self._h, a.ctypes.data_as(C.c_void_p), C.c_uint64(a.size)))
+
+    def getState(self, state):
+        if state.shape[0] < (1 << self._n):
+            raise ValueError("Output array is too small")
+        if state.dtype != self._cdtype or not state.flags.c_contiguous:
+            tmp = np.empty(1 << self._n, dtype=self._cdtype)
+            check(lib.lq_sv_get_data(self._h, tmp.ctypes.data_as(C.c_void_p), C.c_uint64(tmp.size)))
+            state[: tmp.size] = tmp
+        else:
+            check(lib.lq_sv_get_data(self._h, state.ctypes.data_as(C.c_void_p), C.c_uint64(1 << self._n)))
+
+    def state(self) -> np.ndarray:
+        out = np.empty(1 << self._n, dtype=self._cdtype)
+        self.getState(out)
+        return out
+
+    def synchronize(self):
+        check(lib.lq_sv_synchronize(self._h))
+
+    def device_ptr(self) -> int:
+        p = C.c_void_p()
+        check(lib.lq_sv_device_ptr(self._h, C.byref(p)))
+        return p.value
+
+    def profile(self, enable: bool = True):
+        """Time every tile-pass kernel with HIP events on this state's stream."""
+        check(lib.lq_sv_profile(self._h, int(bool(enable))))
+
+    def profile_read(self, reset: bool = True) -> dict:
+        p, ms, b, mx = C.c_uint64(), C.c_double(), C.c_double(), C.c_double()
+        check(lib.lq_sv_profile_read(self._h, int(bool(reset)), C.byref(p), C.byref(ms), C.byref(b), C.byref(mx)))
+        return {"passes": p.value, "ms": ms.value, "bytes": b.value, "max_ms": mx.value}
+
+    def stream_ptr(self) -> int:
+        s = C.c_void_p()
+        check(lib.lq_sv_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    def release_workspace(self):
+        check(lib.lq_sv_release_workspace(self._h))
+
+    def copy_from(self, other):
+        check(lib.lq_sv_copy(self._h, other._h))
+
+    # -- gates ----------------------------------------------------------------
+    def _apply(self, name, wires, inverse=False, params=(), ctrls=(), cvals=()):
+        w, wp, nw = u32(wires)
+        c, cp, nc = u32(ctrls)
+        v, vp = u8(cvals if len(cvals) else [1] * nc)
+        p, pp, np_ = f64(params)
+        check(lib.lq_sv_apply_op(self._h, name.encode(), wp, nw, cp, vp, nc, int(bool(inverse)), pp, np_))
+
+    def applyMatrix(self, matrix, wires, inverse=False):
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=self._cdtype).reshape(-1))
+        w, wp, nw = u32(wires)
+        if m.size != (1 << (2 * nw)):
+            raise LightningException("The size of matrix does not match with the given number of wires")
+        check(lib.lq_sv_apply_matrix(self._h, m.ctypes.data_as(C.c_void_p), wp, nw, None, None, 0,
+                                     int(bool(inverse))))
+
+    def applyControlledMatrix(self, matrix, controlled_wires, controlled_values, wires, inverse=False):
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=self._cdtype).reshape(-1))
+        w, wp, nw = u32(wires)
+        c, cp, nc = u32(controlled_wires)
+        if len(controlled_values) != nc:
+            raise LightningException("`controlled_wires` must have the same size as `controlled_values`.")
+        v, vp = u8(controlled_values)
+        if m.size != (1 << (2 * nw)):
+            raise LightningException("The size of matrix does not match with the given number of wires")
+        check(lib.lq_sv_apply_matrix(self._h, m.ctypes.data_as(C.c_void_p), wp, nw, cp, vp, nc,
+                                     int(bool(inverse))))
+
+    def applyPauliRot(self, wires, inverse, params, word):
+        w, wp, nw = u32(wires)
+        check(lib.lq_sv_apply_pauli_rot(self._h, wp, nw, int(bool(inverse)), float(params[0]), word.encode()))
+
+    def applyGenerator(self, name, wires, adj=False, controlled_wires=(), controlled_values=()):
+        w, wp, nw = u32(wires)
+        c, cp, nc = u32(controlled_wires)
+        v, vp = u8(controlled_values if len(controlled_values) else [1] * nc)
+        s = C.c_double()
+        check(lib.lq_sv_apply_generator(self._h, name.encode(), wp, nw, cp, vp, nc, int(bool(adj)), C.byref(s)))
+        return s.value
+
+    def normalize(self):
+        check(lib.lq_sv_normalize(self._h))
+
+    def collapse(self, wire, branch):
+        check(lib.lq_sv_collapse(self._h, int(wire), int(bool(branch))))
+
+    def kernel_map(self):
+        return {name: "TileLDS" for name in GATE_NAMES}
+
+    def apply_ops(self, ops: "OpsStruct", adjoint: bool = False):
+        check(lib.lq_sv_apply_ops(self._h, ops._h, int(bool(adjoint))))
+
+
+def _make_gate(name):
+    def gate(self, *args, **kwargs):
+        # Gate(wires, inverse, params) or
+        # Gate(controlled_wires, controlled_values, wires, inverse=False, params=[])
+        if len(args) == 3 and _is_bool(args[1]) and not kwargs:
+            wires, inverse, params = args
+            self._apply(name, wires, inverse, params)
+            return
+        if len(args) >= 3:
+            ctrls, cvals, wires = args[0], args[1], args[2]
+            inverse = args[3] if len(args) > 3 else kwargs.get("inverse", False)
+            params = args[4] if len(args) > 4 else kwargs.get("params", [])
+            if len(ctrls) != len(cvals):
+                raise LightningException("`controlled_wires` must have the same size as `controlled_values`.")
+            self._apply(name, wires, inverse, params, ctrls, cvals)
+            return
+        wires = args[0] if args else kwargs["wires"]
+        inverse = args[1] if len(args) > 1 else kwargs.get("inverse", False)
+        params = args[2] if len(args) > 2 else kwargs.get("params", [])
+        self._apply(name, wires, inverse, params)
+
+    gate.__name__ = name
+    gate.__doc__ = f"Apply the {name} gate."
+    return gate
+
+
+for _g in GATE_NAMES:
+    setattr(_StateVectorBase, _g, _make_gate(_g))
+
+
+class StateVectorC64(_StateVectorBase):
+    _bits = 64
+
+
+class StateVectorC128(_StateVectorBase):
+    _bits = 128
+
+
+# ---------------------------------------------------------------------------
+# Observables (core/observables/Observables.hpp:128-584)
+# ---------------------------------------------------------------------------
+class _Obs:
+    _h = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.lq_obs_destroy(h)
+            self._h = None
+
+    def __eq__(self, other):
+        return type(self) is type(other) and self._key() == other._key()
+
+    def __hash__(self):
+        return hash(repr(self))
+
+
+class NamedObs(_Obs):
+    def __init__(self, name: str, wires: Sequence[int]):
+        self._name, self._wires = name, [int(w) for w in wires]
+        w, wp, nw = u32(self._wires)
+        h = C.c_void_p()
+        check(lib.lq_obs_named(name.encode(), wp, nw, C.byref(h)))
+        self._h = h
+
+    def get_wires(self):
+        return list(self._wires)
+
+    def _key(self):
+        return (self._name, tuple(self._wires))
+
+    def __repr__(self):
+        return f"{self._name}{self._wires}"
+
+
+class HermitianObs(_Obs):
+    def __init__(self, matrix, wires: Sequence[int]):
+        self._matrix = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128).reshape(-1))
+        self._wires = [int(w) for w in wires]
+        w, wp, nw = u32(self._wires)
+        h = C.c_void_p()
+        check(lib.lq_obs_hermitian(self._matrix.ctypes.data_as(C.c_void_p), wp, nw, C.byref(h)))
+        self._h = h
+
+    def get_wires(self):
+        return list(self._wires)
+
+    def get_matrix(self):
+        return self._matrix.copy()
+
+    def _key(self):
+        return (tuple(self._matrix.tolist()), tuple(self._wires))
+
+    def __repr__(self):
+        return f"Hermitian{self._wires}"
+
+
+class TensorProdObs(_Obs):
+    def __init__(self, obs: Sequence[_Obs]):
+        self._obs = list(obs)
+        arr = (C.c_void_p * max(len(self._obs), 1))(*[o._h.value for o in self._obs])
+        h = C.c_void_p()
+        check(lib.lq_obs_tensor(arr, len(self._obs), C.byref(h)))
+        self._h = h
+
+    def get_wires(self):
+        return sorted({w for o in self._obs for w in o.get_wires()})
+
+    def get_ops(self):
+        return list(self._obs)
+
+    def _key(self):
+        return tuple(o._key() for o in self._obs)
+
+    def __repr__(self):
+        return " @ ".join(repr(o) for o in self._obs)
+
+
+class Hamiltonian(_Obs):
+    def __init__(self, coeffs, obs: Sequence[_Obs]):
+        self._coeffs = [float(c) for c in np.asarray(coeffs).reshape(-1)]
+        self._obs = list(obs)
+        if len(self._coeffs) != len(self._obs):
+            raise LightningException("coefficients and observables differ in length")
+        c, cp, _ = f64(self._coeffs)
+        arr = (C.c_void_p * max(len(self._obs), 1))(*[o._h.value for o in self._obs])
+        h = C.c_void_p()
+        check(lib.lq_obs_hamiltonian(cp, arr, len(self._obs), C.byref(h)))
+        self._h = h
+
+    def get_wires(self):
+        return sorted({w for o in self._obs for w in o.get_wires()})
+
+    def get_coeffs(self):
+        return list(self._coeffs)
+
+    def get_ops(self):
+        return list(self._obs)
+
+    def _key(self):
+        return (tuple(self._coeffs), tuple(o._key() for o in self._obs))
+
+    def __repr__(self):
+        return "Hamiltonian: { 'coeffs' : " + repr(self._coeffs) + ", 'observables' : " + repr(self._obs) + "}"
+
+
+# ---------------------------------------------------------------------------
+# Measurements (lq/measurements/MeasurementsLQubit.hpp)
+# ---------------------------------------------------------------------------
+class _MeasurementsBase:
+    def __init__(self, sv: _StateVectorBase):
+        self._sv = sv  # the state vector must outlive the measurements, as in the reference
+
+    def probs(self, wires=None):
+        if wires is None:
+            out = np.empty(1 << self._sv._n, dtype=np.float64)
+            w, wp, nw = u32([])
+            check(lib.lq_probs(self._sv._h, wp, 0, 1, out.ctypes.data_as(C.POINTER(C.c_double))))
+        else:
+            w, wp, nw = u32(wires)
+            out = np.empty(1 << nw, dtype=np.float64)
+            check(lib.lq_probs(self._sv._h, wp, nw, 0, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.astype(self._sv._rdtype) if self._sv._bits == 64 else out
+
+    def expval(self, *args):
+        out = C.c_double()
+        if len(args) == 1 and isinstance(args[0], _Obs):
+            check(lib.lq_expval_obs(self._sv._h, args[0]._h, C.byref(out)))
+        elif len(args) == 2 and isinstance(args[0], str):
+            w, wp, nw = u32(args[1])
+            check(lib.lq_expval_named(self._sv._h, args[0].encode(), wp, nw, C.byref(out)))
+        elif len(args) == 2:
+            m = np.ascontiguousarray(np.asarray(args[0], dtype=self._sv._cdtype).reshape(-1))
+            w, wp, nw = u32(args[1])
+            if m.size != (1 << (2 * nw)):
+                raise LightningException("The size of matrix does not match with the given number of wires")
+            check(lib.lq_expval_matrix(self._sv._h, m.ctypes.data_as(C.c_void_p), wp, nw, C.byref(out)))
+        else:
+            raise TypeError("expval(observable), expval(name, wires) or expval(matrix, wires)")
+        return out.value
+
+    def expval_named_batch(self, names: Sequence[str], wires: Sequence[int]) -> np.ndarray:
+        """Several one-wire named expvals from one read of the state."""
+        arr = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+        w, wp, _ = u32(wires)
+        out = np.empty(len(names), dtype=np.float64)
+        check(lib.lq_expval_named_batch(self._sv._h, len(names), arr, wp, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+
+class MeasurementsC64(_MeasurementsBase):
+    pass
+
+
+class MeasurementsC128(_MeasurementsBase):
+    pass
+
+
+# ---------------------------------------------------------------------------
+# Algorithms (core/algorithms/JacobianData.hpp, lq/algorithms/AdjointJacobianLQubit.hpp)
+# ---------------------------------------------------------------------------
+class OpsStruct:
+    """OpsData: names, params, wires, inverses, matrices[, ctrl wires, ctrl values]."""
+
+    def __init__(self, ops_name, ops_params, ops_wires, ops_inverses, ops_matrices,
+                 ops_controlled_wires=None, ops_controlled_values=None):
+        n = len(ops_name)
+        ops_controlled_wires = ops_controlled_wires or [[] for _ in range(n)]
+        ops_controlled_values = ops_controlled_values or [[] for _ in range(n)]
+        self.names = list(ops_name)
+        self.params = [list(map(float, p)) for p in ops_params]
+        self.wires = [list(map(int, w)) for w in ops_wires]
+        self.inverses = [bool(i) for i in ops_inverses]
+        self.matrices = [np.asarray(m, dtype=np.complex128).reshape(-1) if m is not None and len(m) else
+                         np.zeros(0, np.complex128) for m in ops_matrices]
+        self.ctrl_wires = [list(map(int, c)) for c in ops_controlled_wires]
+        self.ctrl_values = [list(map(bool, v)) for v in ops_controlled_values]
+        poff, woff, moff, coff = [0], [0], [0], [0]
+        params, wires, mats, ctrls, cvals = [], [], [], [], []
+        for i in range(n):
+            params += self.params[i]
+            poff.append(len(params))
+            wires += self.wires[i]
+            woff.append(len(wires))
+            m = self.matrices[i]
+            mats += [x for z in m for x in (z.real, z.imag)]
+            moff.append(len(mats) // 2)
+            ctrls += self.ctrl_wires[i]
+            cvals += self.ctrl_values[i]
+            coff.append(len(ctrls))
+        keep = [np.asarray(poff, np.uint32), np.asarray(params + [0.0], np.float64),
+                np.asarray(woff, np.uint32), np.asarray(wires + [0], np.uint32),
+                np.asarray([int(b) for b in self.inverses] + [0], np.uint8), np.asarray(moff, np.uint32),
+                np.asarray(mats + [0.0], np.float64), np.asarray(coff, np.uint32),
+                np.asarray(ctrls + [0], np.uint32), np.asarray([int(b) for b in cvals] + [0], np.uint8)]
+        names = (C.c_char_p * max(n, 1))(*[s.encode() for s in self.names])
+        ptr = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        h = C.c_void_p()
+        check(lib.lq_ops_create(n, names, ptr(keep[0], C.c_uint32), ptr(keep[1], C.c_double),
+                                ptr(keep[2], C.c_uint32), ptr(keep[3], C.c_uint32), ptr(keep[4], C.c_uint8),
+                                ptr(keep[5], C.c_uint32), ptr(keep[6], C.c_double), ptr(keep[7], C.c_uint32),
+                                ptr(keep[8], C.c_uint32), ptr(keep[9], C.c_uint8), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.lq_ops_destroy(h)
+            self._h = None
+
+    def getNumParOps(self):
+        v = C.c_uint64()
+        check(lib.lq_ops_num_par_ops(self._h, C.byref(v)))
+        return v.value
+
+    def __repr__(self):
+        return f"Operations: [{', '.join(self.names)}]"
+
+
+def _create_ops_list(ops_name, ops_params, ops_wires, ops_inverses, ops_matrices,
+                     ops_controlled_wires=None, ops_controlled_values=None):
+    return OpsStruct(ops_name, ops_params, ops_wires, ops_inverses, ops_matrices,
+                     ops_controlled_wires, ops_controlled_values)
+
+
+class AdjointJacobian:
+    """AdjointJacobianC*()(sv, observables, ops, trainable) -> [obs][param] array."""
+
+    def __call__(self, sv: _StateVectorBase, observables, operations: OpsStruct, trainableParams,
+                 apply_operations: bool = False, return_expvals: bool = False):
+        obs = list(observables)
+        tp = np.ascontiguousarray(np.asarray(sorted(int(t) for t in trainableParams), dtype=np.uint64))
+        jac = np.zeros(max(len(obs) * tp.size, 1), dtype=np.float64)
+        ev = np.zeros(max(len(obs), 1), dtype=np.float64)
+        arr = (C.c_void_p * max(len(obs), 1))(*[o._h.value for o in obs])
+        tpp = tp.ctypes.data_as(C.POINTER(C.c_uint64)) if tp.size else None
+        check(lib.lq_adjoint_jacobian(sv._h, operations._h, arr, len(obs), tpp, tp.size, int(bool(apply_operations)),
+                                      jac.ctypes.data_as(C.POINTER(C.c_double)),
+                                      ev.ctypes.data_as(C.POINTER(C.c_double)) if return_expvals else None))
+        out = jac[: len(obs) * tp.size]
+        if sv._bits == 64:
+            out = out.astype(np.float32)
+        if return_expvals:
+            return out, ev[: len(obs)]
+        return out
+
+
+# Precision-suffixed names, as in the reference module
+observables = types.SimpleNamespace()
+algorithms = types.SimpleNamespace()
+for _bits in (64, 128):
+    for _cls in (NamedObs, HermitianObs, TensorProdObs, Hamiltonian):
+        _sub = type(f"{_cls.__name__}C{_bits}", (_cls,), {})
+        setattr(observables, _sub.__name__, _sub)
+        globals()[_sub.__name__] = _sub
+    _ops = type(f"OpsStructC{_bits}", (OpsStruct,), {})
+    _adj = type(f"AdjointJacobianC{_bits}", (AdjointJacobian,), {})
+    setattr(algorithms, _ops.__name__, _ops)
+    setattr(algorithms, _adj.__name__, _adj)
+    setattr(algorithms, f"create_ops_listC{_bits}",
+            (lambda cls: (lambda *a, **k: cls(*a, **k)))(_ops))
+    globals()[_ops.__name__] = _ops
+    globals()[_adj.__name__] = _adj
