@@ -143,6 +143,23 @@ int lq_sv_stream(lq_sv_t sv, void **out);
 /* free the adjoint's work states held by the handle */
 int lq_sv_release_workspace(lq_sv_t sv);
 
+/* ---- Kernel specialisation (no reference counterpart) ----------------------
+ * Register tile passes of states with at least LQ_JIT_MIN_QUBITS qubits
+ * (default 16) are compiled into straight-line kernels in background threads
+ * and cached by structure; until a kernel is ready its pass runs in the
+ * generic tile kernel. LQ_JIT=0 disables this, LQ_JIT_SYNC=1 compiles before
+ * first use. lq_jit_wait blocks until every queued compile is done;
+ * lq_jit_stats reports kernels loaded, compiles failed and compiles pending;
+ * lq_sv_pass_counts reports how many passes of the handle ran specialised and
+ * generic kernels. lq_jit_shutdown drops queued compiles and waits for those
+ * in flight (no new compiles start afterwards); call it before the process
+ * exits when compiles may be in flight -- the Python package does so from
+ * its exit hook. */
+int lq_jit_wait(void);
+int lq_jit_shutdown(void);
+int lq_jit_stats(int *ready, int *failed, int *pending);
+int lq_sv_pass_counts(lq_sv_t sv, int reset, uint64_t *specialised, uint64_t *generic);
+
 #ifdef __cplusplus
 }
 #endif

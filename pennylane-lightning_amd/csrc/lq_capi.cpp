@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "lq_engine.hpp"
+#include "lq_jit.hpp"
 
 using namespace lqamd;
 
@@ -315,6 +316,22 @@ int lq_sv_stream(lq_sv_t sv, void **out) {
 
 int lq_sv_release_workspace(lq_sv_t sv) {
     return guard([&] { S(sv).release_workspace(); });
+}
+
+int lq_jit_wait(void) {
+    return guard([&] { jit_wait(); });
+}
+
+int lq_jit_shutdown(void) {
+    return guard([&] { jit_shutdown(); });
+}
+
+int lq_jit_stats(int *ready, int *failed, int *pending) {
+    return guard([&] { jit_stats(ready, failed, pending); });
+}
+
+int lq_sv_pass_counts(lq_sv_t sv, int reset, uint64_t *specialised, uint64_t *generic) {
+    return guard([&] { S(sv).pass_counts(reset != 0, specialised, generic); });
 }
 
 } // extern "C"

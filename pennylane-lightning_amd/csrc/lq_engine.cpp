@@ -1,8 +1,10 @@
 // Device state vector and executor (host side).
 #include "lq_engine.hpp"
+#include "lq_jit.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -54,6 +56,8 @@ EngineTuning &tuning() {
         if (const char *s = std::getenv("LQ_LOW_BITS")) e.low_bits = std::atoi(s);
         if (const char *s = std::getenv("LQ_PASS_BUDGET")) e.budget = std::atof(s);
         if (const char *s = std::getenv("LQ_LDS_BYTES")) e.lds_bytes = static_cast<size_t>(std::atol(s));
+        if (const char *s = std::getenv("LQ_REG")) e.reg = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_PLAN_DEBUG")) e.plan_debug = std::atoi(s) != 0;
         return e;
     }();
     return t;
@@ -215,6 +219,12 @@ void StateVector::profile(bool on) {
     prof_ = on;
 }
 
+void StateVector::pass_counts(bool reset, uint64_t *specialised, uint64_t *generic) {
+    if (specialised) *specialised = jit_passes_;
+    if (generic) *generic = generic_passes_;
+    if (reset) jit_passes_ = generic_passes_ = 0;
+}
+
 StateVector::PassStats StateVector::profile_read(bool reset) {
     synchronize();
     for (auto &p : pending_) {
@@ -253,7 +263,7 @@ size_t lds_budget() {
         size_t b = tuning().lds_bytes;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lim > 0)
-            b = std::min(b, static_cast<size_t>(lim) - 1024);
+            b = std::min(b, static_cast<size_t>(lim) - 4096);
         return b;
     }();
     return v;
@@ -350,16 +360,40 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     cfg.kt_max = kt_for(ns);
     cfg.low_bits = t.low_bits >= 0 ? t.low_bits : (dtype_ ? 4 : 5);
     cfg.budget = t.budget;
+    // register-resident passes for tiles of 11 or 12 bits with one or two states
+    const int ktr = std::min(cfg.kt_max, 8 + kMaxRegBits);
+    if (t.reg && ns <= 2 && n_ >= 11 && ktr >= 11) {
+        cfg.kt_max = ktr;
+        cfg.reg_rb = ktr - 8;
+        cfg.max_dots = kMaxPassDots;
+    }
     const std::vector<PassPlan> passes = plan_passes(prog, cfg);
+    if (t.plan_debug) {
+        for (size_t p = 0; p < passes.size(); ++p) {
+            const PassPlan &pp = passes[p];
+            int nreg = 0, nlds = 0, ntr = 0;
+            for (const auto &st : pp.stages) {
+                nreg += st.kind == STAGE_REG;
+                nlds += st.kind == STAGE_LDS;
+                ntr += st.kind == STAGE_REG && st.entry == ENTRY_TRANSPOSE;
+            }
+            std::fprintf(stderr, "[lq plan] pass %zu kt=%zu ops=%zu dots=%zu reg=%d stages=%zu (reg %d, lds %d, transposes %d, end %u)\n",
+                         p, pp.tbits.size(), pp.ops.size(), pp.slot_ids.size(), pp.reg ? 1 : 0, pp.stages.size(), nreg,
+                         nlds, ntr, pp.end_transpose);
+        }
+    }
 
     auto align = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
-    std::vector<size_t> op_off(passes.size()), coef_off(passes.size()), part_off(passes.size());
-    size_t ops_bytes = 0, coef_n = 0, part_n = 0;
+    std::vector<size_t> op_off(passes.size()), coef_off(passes.size()), part_off(passes.size()),
+        st_off(passes.size());
+    size_t ops_bytes = 0, coef_n = 0, part_n = 0, st_bytes = 0;
     std::vector<SlotRange> ranges(static_cast<size_t>(nslots), SlotRange{0, 0});
     std::vector<char> seen(static_cast<size_t>(nslots), 0);
     for (size_t p = 0; p < passes.size(); ++p) {
         op_off[p] = ops_bytes;
         ops_bytes += passes[p].ops.size() * sizeof(TileOp);
+        st_off[p] = st_bytes;
+        st_bytes += passes[p].stages.size() * sizeof(TileStage);
         coef_off[p] = coef_n;
         coef_n += passes[p].coef.size();
         part_off[p] = part_n;
@@ -372,7 +406,8 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
         }
         part_n += passes[p].slot_ids.size() * ntiles;
     }
-    const size_t coef_at = align(ops_bytes);
+    const size_t stages_at = align(ops_bytes);
+    const size_t coef_at = align(stages_at + st_bytes);
     const size_t ranges_at = align(coef_at + coef_n * sizeof(double2));
     const size_t total = align(ranges_at + ranges.size() * sizeof(SlotRange));
 
@@ -382,6 +417,8 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     char *hs = static_cast<char *>(staging_.p);
     for (size_t p = 0; p < passes.size(); ++p) {
         std::memcpy(hs + op_off[p], passes[p].ops.data(), passes[p].ops.size() * sizeof(TileOp));
+        if (!passes[p].stages.empty())
+            std::memcpy(hs + stages_at + st_off[p], passes[p].stages.data(), passes[p].stages.size() * sizeof(TileStage));
         double2 *c = reinterpret_cast<double2 *>(hs + coef_at) + coef_off[p];
         for (size_t i = 0; i < passes[p].coef.size(); ++i)
             c[i] = make_double2(passes[p].coef[i].real(), passes[p].coef[i].imag());
@@ -391,6 +428,17 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     hip_check(hipMemcpyAsync(prog_.p, hs, total, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
     hip_check(hipEventRecord(staged_, stream_), "hipEventRecord");
     if (part_n) partials_.ensure(part_n * sizeof(double2));
+
+    // specialised kernels for register passes of large states
+    std::vector<hipFunction_t> fns(passes.size(), nullptr);
+    const JitConfig &jc = jit_config();
+    if (jc.enabled && n_ >= jc.min_qubits) {
+        for (size_t p = 0; p < passes.size(); ++p) {
+            if (!passes[p].reg) continue;
+            const std::string src = jit_pass_source(passes[p], n_, dtype_, ns);
+            if (!src.empty()) fns[p] = jit_lookup(src);
+        }
+    }
 
     char *dp = static_cast<char *>(prog_.p);
     for (size_t p = 0; p < passes.size(); ++p) {
@@ -407,6 +455,24 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
         a.g = make_geom(n_, pp.tbits);
         a.ntiles = 1ull << a.g.nout;
         const size_t lds = static_cast<size_t>(ns) * (elem() << a.g.kt);
+        TileRegArgs ra{};
+        if (pp.reg) {
+            LQ_ABORT_IF(a.g.kt < 11 || a.g.kt > 8 + kMaxRegBits || ns > 2, "register pass geometry out of range");
+            LQ_ABORT_IF(pp.slot_ids.size() > static_cast<size_t>(kMaxPassDots), "too many dots in one pass");
+            for (int s = 0; s < ns; ++s) ra.sv[s] = states[s];
+            ra.ops = a.ops;
+            ra.stages = reinterpret_cast<const TileStage *>(dp + stages_at + st_off[p]);
+            ra.coef = a.coef;
+            ra.partials = a.partials;
+            ra.nstages = static_cast<uint32_t>(pp.stages.size());
+            ra.nslots = static_cast<uint32_t>(pp.slot_ids.size());
+            ra.load_mask = pp.load_mask;
+            ra.store_mask = pp.store_mask;
+            ra.end_transpose = pp.end_transpose;
+            ra.ns = static_cast<uint32_t>(ns);
+            ra.ntiles = a.ntiles;
+            ra.g = a.g;
+        }
         Pending pe{};
         if (prof_) {
             pe.a = take_event();
@@ -415,7 +481,10 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
                        (__builtin_popcount(pp.load_mask) + __builtin_popcount(pp.store_mask));
             hip_check(hipEventRecord(pe.a, stream_), "hipEventRecord");
         }
-        dispatch(dtype_, [&](auto tt) { launch_tile_pass<decltype(tt)>(a, lds, stream_); });
+        if (fns[p]) jit_launch(fns[p], ra.sv[0], ra.sv[1], ra.coef, ra.partials, ra.ntiles, lds, stream_);
+        else if (pp.reg) dispatch(dtype_, [&](auto tt) { launch_tile_reg<decltype(tt)>(ra, lds, stream_); });
+        else dispatch(dtype_, [&](auto tt) { launch_tile_pass<decltype(tt)>(a, lds, stream_); });
+        ++(fns[p] ? jit_passes_ : generic_passes_);
         if (prof_) {
             hip_check(hipEventRecord(pe.b, stream_), "hipEventRecord");
             pending_.push_back(pe);

@@ -79,6 +79,8 @@ struct EngineTuning {
     int low_bits = -1;   // -1: 256-byte segments
     double budget = 24.0;
     size_t lds_bytes = 65536;
+    bool reg = true;     // register-resident passes for large tiles
+    bool plan_debug = false;
 };
 EngineTuning &tuning();
 
@@ -149,6 +151,8 @@ class StateVector {
     };
     void profile(bool on);
     PassStats profile_read(bool reset);
+    // passes launched as specialised kernels / generic tile kernels
+    void pass_counts(bool reset, uint64_t *specialised, uint64_t *generic);
 
     // workspace states (kept between calls; released with release_workspace)
     void *workspace(int i);
@@ -171,6 +175,7 @@ class StateVector {
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> ev_free_;
     PassStats stats_;
+    uint64_t jit_passes_ = 0, generic_passes_ = 0;
     hipEvent_t take_event();
 
     int kt_for(int ns) const;

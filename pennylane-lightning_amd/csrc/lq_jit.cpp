@@ -1,0 +1,682 @@
+// Pass specialisation (see lq_jit.hpp).
+#include "lq_jit.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "lq_common.hpp"
+
+namespace lqamd {
+
+// ---------------------------------------------------------------------------
+// Code generation
+// ---------------------------------------------------------------------------
+namespace {
+
+const char *kPrelude = R"(
+typedef LQ_REAL real;
+typedef LQ_CPLX cplx;
+__device__ __forceinline__ cplx mk(real x, real y) { cplx r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ cplx ldc(const double2 *__restrict__ c, int i) {
+    const double2 d = c[i];
+    return mk((real)d.x, (real)d.y);
+}
+__device__ __forceinline__ cplx cm(const cplx a, const cplx b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cplx cm2(const cplx a, const cplx b, const cplx c, const cplx d) {
+    return mk(a.x * b.x - a.y * b.y + c.x * d.x - c.y * d.y, a.x * b.y + a.y * b.x + c.x * d.y + c.y * d.x);
+}
+__device__ __forceinline__ cplx add(const cplx a, const cplx b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplx ng(const cplx a) { return mk(-a.x, -a.y); }
+__device__ __forceinline__ void d1(cplx &a, cplx &b, const cplx m0, const cplx m1, const cplx m2, const cplx m3) {
+    const cplx x = a, y = b;
+    a = cm2(m0, x, m1, y);
+    b = cm2(m2, x, m3, y);
+}
+__device__ __forceinline__ void d2(cplx &a, cplx &b, cplx &c, cplx &d, const cplx *m) {
+    const cplx x0 = a, x1 = b, x2 = c, x3 = d;
+    a = add(cm2(m[0], x0, m[1], x1), cm2(m[2], x2, m[3], x3));
+    b = add(cm2(m[4], x0, m[5], x1), cm2(m[6], x2, m[7], x3));
+    c = add(cm2(m[8], x0, m[9], x1), cm2(m[10], x2, m[11], x3));
+    d = add(cm2(m[12], x0, m[13], x1), cm2(m[14], x2, m[15], x3));
+}
+__device__ __forceinline__ void acc(double &r, double &i, const cplx b, const cplx v) {
+    const double bx = b.x, by = b.y, vx = v.x, vy = v.y;
+    r += bx * vx + by * vy;
+    i += bx * vy - by * vx;
+}
+__device__ __forceinline__ unsigned pc(unsigned long long x) { return (unsigned)__popcll(x); }
+__device__ __forceinline__ unsigned swz(unsigned l) { return LQ_SWZ; }
+__device__ __forceinline__ void dflush(double r, double i, double2 (*dred)[32], unsigned slot) {
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_xor(r, off, 64);
+        i += __shfl_xor(i, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0) dred[threadIdx.x >> 6][slot] = make_double2(r, i);
+}
+)";
+
+std::string hex64(uint64_t v) {
+    std::ostringstream s;
+    s << "0x" << std::hex << v << "ull";
+    return s.str();
+}
+
+// Expression depositing bits of `src` (bit j -> position pos[j], j < cnt)
+std::string deposit(const char *src, const int *pos, int cnt, bool wide) {
+    std::ostringstream e;
+    bool any = false;
+    for (int j = 0; j < cnt;) {
+        int len = 1;
+        while (j + len < cnt && pos[j + len] == pos[j] + len) ++len;
+        if (any) e << " | ";
+        const uint64_t m = (len >= 64) ? ~0ull : ((1ull << len) - 1);
+        if (wide)
+            e << "((((unsigned long long)" << src << " >> " << j << ") & " << hex64(m) << ") << " << pos[j] << ")";
+        else
+            e << "(((" << src << " >> " << j << ") & " << m << "u) << " << pos[j] << ")";
+        any = true;
+        j += len;
+    }
+    if (!any) return wide ? "0ull" : "0u";
+    return e.str();
+}
+
+struct Mapping {
+    int r[8];
+    int h[8];
+};
+
+class Gen {
+  public:
+    Gen(const PassPlan &p, int n, int dtype, int ns) : p_(p), n_(n), dtype_(dtype), ns_(ns) {
+        kt_ = static_cast<int>(p.tbits.size());
+        rb_ = kt_ - 8;
+        E_ = 1 << rb_;
+        geom_ = make_geom(n, p.tbits);
+        tsize_ = 1u << kt_;
+    }
+
+    std::string run() {
+        std::ostringstream &o = o_;
+        o << "#define LQ_REAL " << (dtype_ ? "double" : "float") << "\n";
+        o << "#define LQ_CPLX " << (dtype_ ? "double2" : "float2") << "\n";
+        o << "#define LQ_SWZ " << (dtype_ ? "(l ^ (((l >> 3) ^ (l >> 6) ^ (l >> 9)) & 7u))" : "(l ^ (((l >> 4) ^ (l >> 8)) & 15u))")
+          << "\n";
+        o << kPrelude;
+        o << "extern \"C\" __global__ __launch_bounds__(256) void lqpass(cplx *__restrict__ sv0, cplx *__restrict__ sv1,"
+             " const double2 *__restrict__ coef, double2 *__restrict__ partials, unsigned long long ntiles) {\n";
+        o << "  extern __shared__ cplx lds[];\n";
+        const int nslots = static_cast<int>(p_.slot_ids.size());
+        if (nslots) o << "  __shared__ double2 dred[4][32];\n";
+        o << "  const unsigned tid = threadIdx.x;\n";
+        o << "  const unsigned long long tile = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * gridDim.x;\n";
+        o << "  if (tile >= ntiles) return;\n";
+        o << "  const unsigned long long base = " << deposit("tile", geom_.ob, geom_.nout, true) << ";\n";
+        if (nslots) {
+            o << "  for (unsigned i = tid; i < 128u; i += 256u) dred[i >> 5][i & 31u] = make_double2(0.0, 0.0);\n";
+            o << "  __syncthreads();\n";
+        }
+        for (int s = 0; s < ns_; ++s) o << "  cplx v" << s << "[" << E_ << "];\n";
+        // load mapping: thread bits -> tile bits 0..7, register bits -> tile bits 8..
+        Mapping m0{};
+        for (int i = 0; i < rb_; ++i) m0.r[i] = 8 + i;
+        for (int j = 0; j < 8; ++j) m0.h[j] = j;
+        cur_ = m0;
+        emit_gl("gl0", m0);
+        for (int s = 0; s < ns_; ++s) {
+            if (!((p_.load_mask >> s) & 1u)) continue;
+            for (int e = 0; e < E_; ++e)
+                o << "  v" << s << "[" << e << "] = sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "];\n";
+        }
+        for (size_t si = 0; si < p_.stages.size(); ++si) {
+            const TileStage &st = p_.stages[si];
+            Mapping m{};
+            for (int i = 0; i < rb_; ++i) m.r[i] = st.rbits[i];
+            for (int j = 0; j < 8; ++j) m.h[j] = st.hbits[j];
+            if (st.entry == ENTRY_TRANSPOSE) transpose(m);
+            cur_ = m;
+            o << "  {\n";
+            emit_gl("gt", m);
+            for (uint32_t oi = st.op_begin; oi < st.op_end; ++oi) emit_op(p_.ops[oi], static_cast<int>(oi));
+            o << "  }\n";
+        }
+        if (p_.store_mask) {
+            if (p_.end_transpose) transpose(m0);
+            cur_ = m0;
+            for (int s = 0; s < ns_; ++s) {
+                if (!((p_.store_mask >> s) & 1u)) continue;
+                for (int e = 0; e < E_; ++e)
+                    o << "  sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "] = v" << s << "[" << e << "];\n";
+            }
+        }
+        if (nslots) {
+            o << "  __syncthreads();\n";
+            o << "  if (tid < " << nslots << "u) {\n";
+            o << "    double2 t = dred[0][tid];\n";
+            o << "    for (int w = 1; w < 4; ++w) { t.x += dred[w][tid].x; t.y += dred[w][tid].y; }\n";
+            o << "    partials[(unsigned long long)tid * ntiles + tile] = t;\n";
+            o << "  }\n";
+        }
+        o << "}\n";
+        return o.str();
+    }
+
+  private:
+    const PassPlan &p_;
+    int n_, dtype_, ns_, kt_, rb_, E_;
+    uint32_t tsize_;
+    TileGeom geom_;
+    Mapping cur_{};
+    bool lds_used_ = false;
+    std::ostringstream o_;
+
+    uint64_t elem_global(const Mapping &m, int e) const {
+        uint64_t g = 0;
+        for (int i = 0; i < rb_; ++i)
+            if ((e >> i) & 1) g |= 1ull << geom_.tb[m.r[i]];
+        return g;
+    }
+    uint32_t elem_local(const Mapping &m, int e) const {
+        uint32_t l = 0;
+        for (int i = 0; i < rb_; ++i)
+            if ((e >> i) & 1) l |= 1u << m.r[i];
+        return l;
+    }
+    uint64_t regmask() const {
+        uint64_t g = 0;
+        for (int i = 0; i < rb_; ++i) g |= 1ull << geom_.tb[cur_.r[i]];
+        return g;
+    }
+    int reg_index_of_global(int gb) const {
+        for (int i = 0; i < rb_; ++i)
+            if (geom_.tb[cur_.r[i]] == gb) return i;
+        return -1;
+    }
+    void emit_gl(const char *name, const Mapping &m) {
+        int pos[8];
+        for (int j = 0; j < 8; ++j) pos[j] = geom_.tb[m.h[j]];
+        o_ << "  const unsigned long long " << name << " = base | " << deposit("tid", pos, 8, true) << ";\n";
+    }
+    void transpose(const Mapping &to) {
+        std::ostringstream &o = o_;
+        if (lds_used_) o << "  __syncthreads();\n";
+        o << "  {\n";
+        o << "    const unsigned lt = " << deposit("tid", cur_.h, 8, false) << ";\n";
+        for (int s = 0; s < ns_; ++s) {
+            if (!((p_.load_mask >> s) & 1u)) continue;
+            for (int e = 0; e < E_; ++e)
+                o << "    lds[" << s * tsize_ << "u + swz(lt | " << elem_local(cur_, e) << "u)] = v" << s << "[" << e
+                  << "];\n";
+        }
+        o << "  }\n  __syncthreads();\n  {\n";
+        o << "    const unsigned lt = " << deposit("tid", to.h, 8, false) << ";\n";
+        for (int s = 0; s < ns_; ++s) {
+            if (!((p_.load_mask >> s) & 1u)) continue;
+            for (int e = 0; e < E_; ++e)
+                o << "    v" << s << "[" << e << "] = lds[" << s * tsize_ << "u + swz(lt | " << elem_local(to, e) << "u)];\n";
+        }
+        o << "  }\n";
+        lds_used_ = true;
+    }
+
+    // split a (mask, value) condition on global bits into the register part
+    // (checked per element at generation time) and the thread part
+    bool elem_ok(uint64_t cm, uint64_t cv, int e) const {
+        for (int i = 0; i < rb_; ++i) {
+            const int gb = geom_.tb[cur_.r[i]];
+            if ((cm >> gb) & 1ull)
+                if (static_cast<uint64_t>((e >> i) & 1) != ((cv >> gb) & 1ull)) return false;
+        }
+        return true;
+    }
+    // returns the guard expression for the thread part ("" when none)
+    std::string thread_guard(uint64_t cm, uint64_t cv) const {
+        const uint64_t tm = cm & ~regmask();
+        if (!tm) return "";
+        return "((gt & " + hex64(tm) + ") == " + hex64(cv & tm) + ")";
+    }
+
+    std::vector<int> states_of(uint32_t smask) const {
+        std::vector<int> v;
+        for (int s = 0; s < ns_; ++s)
+            if ((smask >> s) & 1u) v.push_back(s);
+        return v;
+    }
+
+    void emit_op(const TileOp &t, int id) {
+        std::ostringstream &o = o_;
+        const bool dot = (t.flags & TF_DOT) != 0;
+        const std::string guard = thread_guard(t.gcmask, t.gcval);
+        const auto sts = states_of(t.smask);
+        const std::string A = "v" + std::to_string(t.sa), B = "v" + std::to_string(t.sb);
+        o << "  { // op " << id << " kind " << t.kind << "\n";
+        if (dot) o << "    double ar = 0.0, ai = 0.0;\n";
+        const std::string open = guard.empty() ? "" : "    if (" + guard + ") {\n";
+        const std::string close = guard.empty() ? "" : "    }\n";
+        const uint64_t off = t.coef_off;
+        switch (t.kind) {
+        case TOP_DENSE1: {
+            const int I = t.lpos[0];
+            o << "    const cplx m0 = ldc(coef, " << off << "), m1 = ldc(coef, " << off + 1 << "), m2 = ldc(coef, " << off + 2
+              << "), m3 = ldc(coef, " << off + 3 << ");\n";
+            o << open;
+            for (int e = 0; e < E_; ++e) {
+                if ((e >> I) & 1) continue;
+                if (!elem_ok(t.gcmask, t.gcval, e)) continue;
+                const int e1 = e | (1 << I);
+                if (dot) {
+                    o << "    acc(ar, ai, " << B << "[" << e << "], cm2(m0, " << A << "[" << e << "], m1, " << A << "[" << e1
+                      << "]));\n";
+                    o << "    acc(ar, ai, " << B << "[" << e1 << "], cm2(m2, " << A << "[" << e << "], m3, " << A << "["
+                      << e1 << "]));\n";
+                } else {
+                    for (int s : sts) o << "    d1(v" << s << "[" << e << "], v" << s << "[" << e1 << "], m0, m1, m2, m3);\n";
+                }
+            }
+            o << close;
+            break;
+        }
+        case TOP_DENSE2: {
+            const int I0 = t.lpos[0], I1 = t.lpos[1];
+            o << "    cplx M[16];\n";
+            for (int i = 0; i < 16; ++i) o << "    M[" << i << "] = ldc(coef, " << off + i << ");\n";
+            o << open;
+            for (int e = 0; e < E_; ++e) {
+                if (((e >> I0) & 1) || ((e >> I1) & 1)) continue;
+                if (!elem_ok(t.gcmask, t.gcval, e)) continue;
+                const int idx[4] = {e, e | (1 << I1), e | (1 << I0), e | (1 << I0) | (1 << I1)};
+                if (dot) {
+                    for (int r = 0; r < 4; ++r) {
+                        o << "    acc(ar, ai, " << B << "[" << idx[r] << "], add(cm2(M[" << 4 * r << "], " << A << "["
+                          << idx[0] << "], M[" << 4 * r + 1 << "], " << A << "[" << idx[1] << "]), cm2(M[" << 4 * r + 2
+                          << "], " << A << "[" << idx[2] << "], M[" << 4 * r + 3 << "], " << A << "[" << idx[3] << "])));\n";
+                    }
+                } else {
+                    for (int s : sts)
+                        o << "    d2(v" << s << "[" << idx[0] << "], v" << s << "[" << idx[1] << "], v" << s << "["
+                          << idx[2] << "], v" << s << "[" << idx[3] << "], M);\n";
+                }
+            }
+            o << close;
+            break;
+        }
+        case TOP_DIAG:
+        case TOP_PCPHASE:
+        case TOP_PARITY: emit_phase(t, sts, dot, A, B, open, close); break;
+        case TOP_SIGN: {
+            o << open;
+            for (int e = 0; e < E_; ++e) {
+                if (!elem_ok(t.gcmask, t.gcval, e)) continue;
+                for (int s : sts) o << "    v" << s << "[" << e << "] = ng(v" << s << "[" << e << "]);\n";
+            }
+            o << close;
+            break;
+        }
+        case TOP_CZSET: emit_czset(t, sts); break;
+        default: LQ_ABORT("kernel generator: unsupported op");
+        }
+        if (dot) o << "    dflush(ar, ai, dred, " << t.slot << "u);\n";
+        o << "  }\n";
+    }
+
+    // DIAG (k <= 2 in register passes), PCPHASE, PARITY: a phase per element
+    // chosen by bits of its global index.
+    void emit_phase(const TileOp &t, const std::vector<int> &sts, bool dot, const std::string &A,
+                    const std::string &B, const std::string &open, const std::string &close) {
+        std::ostringstream &o = o_;
+        const uint64_t off = t.coef_off;
+        // per element: phase index = thread part (runtime) | register part (constant)
+        std::vector<std::string> phase_of(E_);
+        if (t.kind == TOP_PARITY) {
+            const uint64_t tm = t.gpmask & ~regmask();
+            o << "    const cplx d0 = ldc(coef, " << off << "), d1_ = ldc(coef, " << off + 1 << ");\n";
+            if (tm) {
+                o << "    const bool pt = (pc(gt & " << hex64(tm) << ") & 1u) != 0u;\n";
+                o << "    const cplx q0 = pt ? d1_ : d0, q1 = pt ? d0 : d1_;\n";
+            } else {
+                o << "    const cplx q0 = d0, q1 = d1_;\n";
+            }
+            for (int e = 0; e < E_; ++e) {
+                int par = 0;
+                for (int i = 0; i < rb_; ++i)
+                    if (((t.gpmask >> geom_.tb[cur_.r[i]]) & 1ull) && ((e >> i) & 1)) par ^= 1;
+                phase_of[e] = par ? "q1" : "q0";
+            }
+        } else {
+            const int k = static_cast<int>(t.k);
+            // thread-part index
+            std::ostringstream ti;
+            bool any = false;
+            int reg_of[kMaxTileOpTargets];
+            for (int j = 0; j < k; ++j) {
+                reg_of[j] = reg_index_of_global(t.gpos[j]);
+                if (reg_of[j] < 0) {
+                    if (any) ti << " | ";
+                    ti << "((unsigned)((gt >> " << t.gpos[j] << ") & 1ull) << " << (k - 1 - j) << ")";
+                    any = true;
+                }
+            }
+            o << "    const unsigned ti = " << (any ? ti.str() : "0u") << ";\n";
+            const int nd = t.kind == TOP_PCPHASE ? 2 : (1 << k);
+            for (int i = 0; i < nd; ++i) o << "    const cplx d" << i << " = ldc(coef, " << off + i << ");\n";
+            std::vector<int> done(1 << std::max(k, 1), 0);
+            for (int e = 0; e < E_; ++e) {
+                uint32_t rpart = 0;
+                for (int j = 0; j < k; ++j)
+                    if (reg_of[j] >= 0 && ((e >> reg_of[j]) & 1)) rpart |= 1u << (k - 1 - j);
+                const std::string name = "p" + std::to_string(rpart);
+                if (!done[rpart]) {
+                    done[rpart] = 1;
+                    if (t.kind == TOP_PCPHASE) {
+                        o << "    const cplx " << name << " = ((ti | " << rpart << "u) < " << t.pc_dim << "ull) ? d0 : d1;\n";
+                    } else if (!any) {
+                        o << "    const cplx " << name << " = d" << rpart << ";\n";
+                    } else {
+                        o << "    const unsigned x" << rpart << " = ti | " << rpart << "u;\n";
+                        o << "    const cplx " << name << " = ";
+                        for (int i = 0; i < nd - 1; ++i) o << "(x" << rpart << " == " << i << "u) ? d" << i << " : ";
+                        o << "d" << nd - 1 << ";\n";
+                    }
+                }
+                phase_of[e] = name;
+            }
+        }
+        o << open;
+        for (int e = 0; e < E_; ++e) {
+            if (!elem_ok(t.gcmask, t.gcval, e)) continue;
+            if (dot) {
+                o << "    acc(ar, ai, " << B << "[" << e << "], cm(" << phase_of[e] << ", " << A << "[" << e << "]));\n";
+            } else {
+                for (int s : sts)
+                    o << "    v" << s << "[" << e << "] = cm(" << phase_of[e] << ", v" << s << "[" << e << "]);\n";
+            }
+        }
+        o << close;
+    }
+
+    // CZ runs: negate where the number of set (a, a + d) pairs is odd. Pairs
+    // inside the register bits are resolved per element at generation time;
+    // pairs with one register bit contribute that bit times a per-thread
+    // parity; pairs outside the register bits give a per-thread parity.
+    void emit_czset(const TileOp &t, const std::vector<int> &sts) {
+        std::ostringstream &o = o_;
+        const uint64_t rm = regmask();
+        std::vector<uint64_t> tmask(rb_, 0);        // thread bits paired with register bit i
+        std::vector<std::pair<uint32_t, uint64_t>> tt; // (d, mask of low bits a) both in thread part
+        std::vector<uint32_t> qpairs;                // register-register pairs as (i << 8 | j)
+        for (uint32_t z = 0; z < t.nz; ++z) {
+            const uint32_t d = t.sd[z];
+            uint64_t ttm = 0;
+            for (int a = 0; a < 64; ++a) {
+                if (!((t.sm[z] >> a) & 1ull)) continue;
+                const int b = a + static_cast<int>(d);
+                const bool ra = (rm >> a) & 1ull, rbb = (rm >> b) & 1ull;
+                if (ra && rbb) qpairs.push_back((reg_index_of_global(a) << 8) | reg_index_of_global(b));
+                else if (ra) tmask[reg_index_of_global(a)] |= 1ull << b;
+                else if (rbb) tmask[reg_index_of_global(b)] |= 1ull << a;
+                else ttm |= 1ull << a;
+            }
+            if (ttm) tt.emplace_back(d, ttm);
+        }
+        o << "    const unsigned p0 = (0u";
+        for (const auto &x : tt) o << " + pc(gt & (gt >> " << x.first << ") & " << hex64(x.second) << ")";
+        o << ") & 1u;\n";
+        for (int i = 0; i < rb_; ++i)
+            if (tmask[i]) o << "    const unsigned l" << i << " = pc(gt & " << hex64(tmask[i]) << ") & 1u;\n";
+        for (int e = 0; e < E_; ++e) {
+            int q = 0;
+            for (uint32_t pr : qpairs)
+                if (((e >> (pr >> 8)) & 1) && ((e >> (pr & 255)) & 1)) q ^= 1;
+            std::ostringstream cond;
+            cond << "(p0";
+            for (int i = 0; i < rb_; ++i)
+                if (tmask[i] && ((e >> i) & 1)) cond << " ^ l" << i;
+            cond << (q ? " ^ 1u" : "") << ")";
+            for (int s : sts)
+                o << "    if (" << cond.str() << ") v" << s << "[" << e << "] = ng(v" << s << "[" << e << "]);\n";
+        }
+    }
+};
+} // namespace
+
+std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns) {
+    if (!p.reg || p.stages.empty()) return "";
+    const int kt = static_cast<int>(p.tbits.size());
+    if (kt < 11 || kt > 13 || ns > 2 || p.slot_ids.size() > 32) return "";
+    for (const auto &st : p.stages)
+        if (st.kind != STAGE_REG) return "";
+    for (const auto &t : p.ops) {
+        switch (t.kind) {
+        case TOP_DENSE1:
+        case TOP_DENSE2:
+        case TOP_PARITY:
+        case TOP_SIGN:
+        case TOP_CZSET: break;
+        case TOP_DIAG:
+            if (t.k > 2) return "";
+            break;
+        case TOP_PCPHASE:
+            if (t.k > 12) return "";
+            break;
+        default: return "";
+        }
+        if (t.flags & TF_ZERO_UNCTRL) return "";
+    }
+    Gen g(p, n, dtype, ns);
+    return g.run();
+}
+
+// ---------------------------------------------------------------------------
+// Compile service
+// ---------------------------------------------------------------------------
+JitConfig &jit_config() {
+    static JitConfig c = [] {
+        JitConfig j;
+        if (const char *s = std::getenv("LQ_JIT")) j.enabled = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_JIT_SYNC")) j.sync = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_JIT_MIN_QUBITS")) j.min_qubits = std::atoi(s);
+        if (const char *s = std::getenv("LQ_JIT_THREADS")) j.threads = std::max(1, std::atoi(s));
+        return j;
+    }();
+    return c;
+}
+
+namespace {
+enum class JState { Pending, Ready, Failed };
+
+struct JEntry {
+    JState state = JState::Pending;
+    std::vector<char> code;
+    std::string log;
+    std::unordered_map<int, std::pair<hipModule_t, hipFunction_t>> loaded; // per device
+};
+
+class Service {
+  public:
+    static Service &get() {
+        static Service *s = new Service(); // lives until exit; see stop()
+        return *s;
+    }
+
+    hipFunction_t lookup(const std::string &src) {
+        std::shared_ptr<JEntry> e;
+        bool compile_now = false;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            auto it = map_.find(src);
+            if (it == map_.end()) {
+                if (stopping_) return nullptr;
+                e = std::make_shared<JEntry>();
+                map_.emplace(src, e);
+                if (jit_config().sync) {
+                    compile_now = true;
+                    ++running_;
+                } else {
+                    queue_.push_back(src);
+                    ++pending_;
+                    start_workers();
+                    cv_.notify_one();
+                    return nullptr;
+                }
+            } else {
+                e = it->second;
+            }
+        }
+        if (compile_now) {
+            compile(src, *e);
+            std::lock_guard<std::mutex> lk(mu_);
+            --running_;
+            done_cv_.notify_all();
+        }
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+        std::lock_guard<std::mutex> lk(mu_);
+        if (e->state != JState::Ready) return nullptr;
+        auto it = e->loaded.find(dev);
+        if (it != e->loaded.end()) return it->second.second;
+        hipModule_t mod = nullptr;
+        hipFunction_t fn = nullptr;
+        if (hipModuleLoadData(&mod, e->code.data()) != hipSuccess ||
+            hipModuleGetFunction(&fn, mod, "lqpass") != hipSuccess) {
+            e->state = JState::Failed;
+            ++failed_;
+            return nullptr;
+        }
+        e->loaded.emplace(dev, std::make_pair(mod, fn));
+        ++ready_;
+        return fn;
+    }
+
+    void wait() {
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [this] { return pending_ == 0 && running_ == 0; });
+    }
+
+    void stats(int *ready, int *failed, int *pending) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (ready) *ready = ready_;
+        if (failed) *failed = failed_;
+        if (pending) *pending = pending_ + running_;
+    }
+
+    // Drop queued work and wait for compiles in flight; later lookups find
+    // only kernels already compiled. Runs at process exit at the latest.
+    void stop() {
+        std::unique_lock<std::mutex> lk(mu_);
+        stopping_ = true;
+        pending_ -= static_cast<int>(queue_.size());
+        queue_.clear();
+        cv_.notify_all();
+        done_cv_.wait(lk, [this] { return running_ == 0; });
+    }
+
+  private:
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::unordered_map<std::string, std::shared_ptr<JEntry>> map_;
+    std::deque<std::string> queue_;
+    int pending_ = 0, running_ = 0, ready_ = 0, failed_ = 0;
+    bool started_ = false, stopping_ = false;
+
+    std::once_flag exit_hook_;
+
+    Service() = default;
+
+    // The compiler's libraries are loaded by the first compile and register
+    // their own exit-time destructors then; a hook registered after them runs
+    // before them (exit handlers run in reverse order), so no compile is still
+    // inside those libraries when they are torn down.
+    void hook_exit() {
+        std::call_once(exit_hook_, [] { std::atexit([] { Service::get().stop(); }); });
+    }
+
+    void start_workers() {
+        if (started_) return;
+        started_ = true;
+        for (int i = 0; i < jit_config().threads; ++i) std::thread([this] { worker(); }).detach();
+    }
+
+    void worker() {
+        while (true) {
+            std::string src;
+            std::shared_ptr<JEntry> e;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return stopping_ || !queue_.empty(); });
+                if (stopping_) return;
+                src = queue_.front();
+                queue_.pop_front();
+                e = map_[src];
+                --pending_;
+                ++running_;
+            }
+            compile(src, *e);
+            std::lock_guard<std::mutex> lk(mu_);
+            --running_;
+            done_cv_.notify_all();
+        }
+    }
+
+    static void compile(const std::string &src, JEntry &e) {
+        hiprtcProgram prog;
+        JState st = JState::Failed;
+        std::vector<char> code;
+        std::string log;
+        if (hiprtcCreateProgram(&prog, src.c_str(), "lq_pass.hip", 0, nullptr, nullptr) == HIPRTC_SUCCESS) {
+            const char *opts[] = {"--offload-arch=gfx950", "-O3",        "-ffp-contract=fast", "-Xclang",
+                                  "-target-feature",       "-Xclang",    "-packed-fp32-ops"};
+            const hiprtcResult r = hiprtcCompileProgram(prog, 7, opts);
+            size_t ls = 0;
+            if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+                log.resize(ls);
+                hiprtcGetProgramLog(prog, &log[0]);
+            }
+            size_t cs = 0;
+            if (r == HIPRTC_SUCCESS && hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS && cs > 0) {
+                code.resize(cs);
+                if (hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS) st = JState::Ready;
+            }
+            hiprtcDestroyProgram(&prog);
+        }
+        get().hook_exit();
+        std::lock_guard<std::mutex> lk(get().mu_);
+        e.code.swap(code);
+        e.log.swap(log);
+        e.state = st;
+        if (st == JState::Failed) ++get().failed_;
+    }
+};
+} // namespace
+
+hipFunction_t jit_lookup(const std::string &src) { return Service::get().lookup(src); }
+void jit_wait() { Service::get().wait(); }
+void jit_shutdown() { Service::get().stop(); }
+void jit_stats(int *ready, int *failed, int *pending) { Service::get().stats(ready, failed, pending); }
+
+void jit_launch(hipFunction_t f, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
+                size_t lds_bytes, hipStream_t s) {
+    void *a0 = sv0, *a1 = sv1, *a3 = partials;
+    const void *a2 = coef;
+    unsigned long long nt = ntiles;
+    void *args[] = {&a0, &a1, &a2, &a3, &nt};
+    const uint32_t gx = static_cast<uint32_t>(ntiles < 65536 ? ntiles : 65536);
+    const uint32_t gy = static_cast<uint32_t>((ntiles + gx - 1) / gx);
+    const hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, kBlock, 1, 1, static_cast<unsigned>(lds_bytes), s, args, nullptr);
+    if (e != hipSuccess) {
+        const std::string m = std::string("hipModuleLaunchKernel: ") + hipGetErrorString(e);
+        LQ_ABORT(m.c_str());
+    }
+}
+
+} // namespace lqamd

@@ -1,0 +1,45 @@
+// Pass specialisation: a planned register pass becomes straight-line HIP
+// source (register bits, control splits, CZ parities and tile geometry as
+// compile-time constants; coefficients stay in the runtime buffer), compiled
+// with hiprtc for gfx950 and cached by its source text. A circuit structure
+// that repeats (variational loops, the adjoint sweep of a fixed ansatz, the
+// benchmark) runs the specialised kernels; until a kernel is compiled the
+// pass runs in the generic tile kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "lq_plan.hpp"
+
+namespace lqamd {
+
+// Source for a register pass of n qubits, dtype 0/1 (complex64/128), ns
+// states; empty when the pass has stages the generator does not cover.
+std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns);
+
+struct JitConfig {
+    bool enabled = true;
+    bool sync = false;      // compile before first use instead of in the background
+    int min_qubits = 16;    // smaller states always use the generic kernels
+    int threads = 8;        // background compile threads
+};
+JitConfig &jit_config();
+
+// Kernel for `src` if compiled and loaded on the current device; otherwise
+// nullptr, after queueing the compile (or compiling now in sync mode).
+hipFunction_t jit_lookup(const std::string &src);
+// Block until every queued compile has finished.
+void jit_wait();
+// Drop queued compiles and wait for those in flight; no new compiles start
+// afterwards (kernels already compiled stay usable).
+void jit_shutdown();
+// Counters: kernels ready, compiles failed, compiles pending.
+void jit_stats(int *ready, int *failed, int *pending);
+
+// Launch a specialised pass (arguments as the generic register kernel).
+void jit_launch(hipFunction_t f, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
+                size_t lds_bytes, hipStream_t s);
+
+} // namespace lqamd

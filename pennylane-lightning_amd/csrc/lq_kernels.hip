@@ -7,57 +7,20 @@
 // double, also for complex64 states.
 #include <hip/hip_runtime.h>
 
+#include "lq_devcommon.hpp"
 #include "lq_kernels.hpp"
 
 namespace lqamd {
 namespace {
 
-template <typename T> struct VecT;
-template <> struct VecT<double> { using type = double2; };
-template <> struct VecT<float> { using type = float2; };
-
-template <typename T> __device__ __forceinline__ typename VecT<T>::type mk(T x, T y) {
-    typename VecT<T>::type v;
-    v.x = x;
-    v.y = y;
-    return v;
-}
-
-template <typename T> __device__ __forceinline__ typename VecT<T>::type ld_coef(const double2 *c, uint64_t i) {
-    const double2 d = c[i];
-    return mk<T>(static_cast<T>(d.x), static_cast<T>(d.y));
-}
-
-// a*b
-template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
-    V r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
-    return r;
-}
-// a*b + c*d
-template <typename V> __device__ __forceinline__ V cmul2(V a, V b, V c, V d) {
-    V r;
-    r.x = a.x * b.x - a.y * b.y + c.x * d.x - c.y * d.y;
-    r.y = a.x * b.y + a.y * b.x + c.x * d.y + c.y * d.x;
-    return r;
-}
-// acc += conj(b) * v, in double
-template <typename V> __device__ __forceinline__ void acc_dot(double &re, double &im, V b, V v) {
-    const double bx = b.x, by = b.y, vx = v.x, vy = v.y;
-    re += bx * vx + by * vy;
-    im += bx * vy - by * vx;
-}
-
-__device__ __forceinline__ uint32_t insert_zero(uint32_t p, int q) {
-    return ((p >> q) << (q + 1)) | (p & ((1u << q) - 1u));
-}
+using dev::VecT;
+using dev::mk;
+using dev::ld_coef;
+using dev::cmul;
+using dev::acc_dot;
 
 __device__ __forceinline__ void block_reduce_store(double re, double im, double2 *red, double2 *dst) {
-    for (int off = 32; off > 0; off >>= 1) {
-        re += __shfl_xor(re, off, 64);
-        im += __shfl_xor(im, off, 64);
-    }
+    dev::wave_sum(re, im);
     const uint32_t tid = threadIdx.x;
     if ((tid & 63u) == 0) red[tid >> 6] = make_double2(re, im);
     __syncthreads();
@@ -72,7 +35,9 @@ __device__ __forceinline__ void block_reduce_store(double re, double im, double2
 }
 
 // ---------------------------------------------------------------------------
-// Tile-program kernel
+// Small-tile kernel: the whole tile lives in LDS and every op sweeps it.
+// Used for tiles of at most 2^10 amplitudes and for passes with more than two
+// states; larger tiles run in tile_reg_kernel (lq_kernels_reg.hip).
 // ---------------------------------------------------------------------------
 template <typename T, int NS>
 __global__ __launch_bounds__(kBlock) void tile_pass_kernel(TilePassArgs a) {
@@ -115,264 +80,23 @@ __global__ __launch_bounds__(kBlock) void tile_pass_kernel(TilePassArgs a) {
     const V zero = mk<T>(T(0), T(0));
     for (uint32_t oi = 0; oi < a.nops; ++oi) {
         const TileOp &op = a.ops[oi];
-        const uint32_t kind = op.kind;
         const bool dot = (op.flags & TF_DOT) != 0;
-        const bool zero_unctrl = (op.flags & TF_ZERO_UNCTRL) != 0;
-        const uint32_t smask = op.smask;
         double2 *dst = dot ? &a.partials[static_cast<uint64_t>(op.slot) * a.ntiles + tile] : nullptr;
-
         if ((base & op.ocmask) != op.ocval) {
             // control bits outside the tile do not match: no-op for this tile
             if (dot) {
                 if (tid == 0) *dst = make_double2(0.0, 0.0);
-            } else if (zero_unctrl) {
+            } else if (op.flags & TF_ZERO_UNCTRL) {
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    if ((smask >> s) & 1u)
+                    if ((op.smask >> s) & 1u)
                         for (uint32_t l = tid; l < tsize; l += kBlock) lds[s * tsize + l] = zero;
             }
             __syncthreads();
             continue;
         }
-        const uint32_t lcm = op.lcmask, lcv = op.lcval;
-        const V *A = lds + op.sa * tsize;
-        const V *B = lds + op.sb * tsize;
         double accr = 0.0, acci = 0.0;
-
-        if (kind == TOP_DENSE1) {
-            const int q = op.lpos[0];
-            const uint32_t qb = 1u << q;
-            const V m00 = ld_coef<T>(a.coef, op.coef_off + 0), m01 = ld_coef<T>(a.coef, op.coef_off + 1);
-            const V m10 = ld_coef<T>(a.coef, op.coef_off + 2), m11 = ld_coef<T>(a.coef, op.coef_off + 3);
-            for (uint32_t p = tid; p < (tsize >> 1); p += kBlock) {
-                const uint32_t i0 = insert_zero(p, q), i1 = i0 | qb;
-                const bool ok = (i0 & lcm) == lcv;
-                if (dot) {
-                    if (ok) {
-                        const V a0 = A[i0], a1 = A[i1];
-                        acc_dot(accr, acci, B[i0], cmul2(m00, a0, m01, a1));
-                        acc_dot(accr, acci, B[i1], cmul2(m10, a0, m11, a1));
-                    }
-                } else {
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (!((smask >> s) & 1u)) continue;
-                        V *S = lds + s * tsize;
-                        if (ok) {
-                            const V a0 = S[i0], a1 = S[i1];
-                            S[i0] = cmul2(m00, a0, m01, a1);
-                            S[i1] = cmul2(m10, a0, m11, a1);
-                        } else if (zero_unctrl) {
-                            S[i0] = zero;
-                            S[i1] = zero;
-                        }
-                    }
-                }
-            }
-        } else if (kind == TOP_DENSE2) {
-            const int q0 = op.lpos[0], q1 = op.lpos[1];
-            const int lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-            const uint32_t b0 = 1u << q0, b1 = 1u << q1;
-            V m[16];
-#pragma unroll
-            for (int e = 0; e < 16; ++e) m[e] = ld_coef<T>(a.coef, op.coef_off + e);
-            for (uint32_t p = tid; p < (tsize >> 2); p += kBlock) {
-                const uint32_t i00 = insert_zero(insert_zero(p, lo), hi);
-                const uint32_t idx[4] = {i00, i00 | b1, i00 | b0, i00 | b0 | b1};
-                const bool ok = (i00 & lcm) == lcv;
-                if (dot) {
-                    if (ok) {
-                        V v[4];
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) v[e] = A[idx[e]];
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const V o = cmul2(m[4 * r], v[0], m[4 * r + 1], v[1]);
-                            const V o2 = cmul2(m[4 * r + 2], v[2], m[4 * r + 3], v[3]);
-                            acc_dot(accr, acci, B[idx[r]], mk<T>(o.x + o2.x, o.y + o2.y));
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (!((smask >> s) & 1u)) continue;
-                        V *S = lds + s * tsize;
-                        if (ok) {
-                            V v[4];
-#pragma unroll
-                            for (int e = 0; e < 4; ++e) v[e] = S[idx[e]];
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const V o = cmul2(m[4 * r], v[0], m[4 * r + 1], v[1]);
-                                const V o2 = cmul2(m[4 * r + 2], v[2], m[4 * r + 3], v[3]);
-                                S[idx[r]] = mk<T>(o.x + o2.x, o.y + o2.y);
-                            }
-                        } else if (zero_unctrl) {
-#pragma unroll
-                            for (int e = 0; e < 4; ++e) S[idx[e]] = zero;
-                        }
-                    }
-                }
-            }
-        } else if (kind == TOP_DENSEK) {
-            const int k = static_cast<int>(op.k);
-            const uint32_t dim = 1u << k;
-            int srt[kMaxTileOpTargets];
-            for (int j = 0; j < k; ++j) srt[j] = op.lpos[j];
-            for (int i = 1; i < k; ++i) // insertion sort, uniform
-                for (int j = i; j > 0 && srt[j - 1] > srt[j]; --j) {
-                    const int t = srt[j];
-                    srt[j] = srt[j - 1];
-                    srt[j - 1] = t;
-                }
-            auto lsp = [&](uint32_t col) -> uint32_t {
-                uint32_t o = 0;
-                for (int j = 0; j < k; ++j) o |= ((col >> (k - 1 - j)) & 1u) << op.lpos[j];
-                return o;
-            };
-            const uint32_t R = dim > static_cast<uint32_t>(kBlock) ? dim / kBlock : 1u;
-            const uint32_t per_round = kBlock * R;
-            const uint32_t rounds = tsize > per_round ? tsize / per_round : 1u;
-            const double2 *M = a.coef + op.coef_off;
-            for (int s = 0; s < (dot ? 1 : NS); ++s) {
-                if (!dot && !((smask >> s) & 1u)) continue;
-                V *S = dot ? const_cast<V *>(A) : lds + s * tsize;
-                for (uint32_t r = 0; r < rounds; ++r) {
-                    V outv[4];
-                    uint32_t outi[4];
-                    bool outok[4];
-                    for (uint32_t rr = 0; rr < R; ++rr) {
-                        const uint32_t e = r * per_round + rr * kBlock + tid;
-                        outok[rr] = false;
-                        outi[rr] = 0;
-                        outv[rr] = zero;
-                        if (e >= tsize) continue;
-                        const uint32_t g = e >> k, row = e & (dim - 1u);
-                        uint32_t gb = g;
-                        for (int j = 0; j < k; ++j) gb = insert_zero(gb, srt[j]);
-                        outi[rr] = gb | lsp(row);
-                        if ((gb & lcm) != lcv) continue;
-                        outok[rr] = true;
-                        double ox = 0.0, oy = 0.0;
-                        for (uint32_t col = 0; col < dim; ++col) {
-                            const double2 mm = M[static_cast<uint64_t>(row) * dim + col];
-                            const V v = S[gb | lsp(col)];
-                            const T mx = static_cast<T>(mm.x), my = static_cast<T>(mm.y);
-                            ox += static_cast<double>(mx * v.x - my * v.y);
-                            oy += static_cast<double>(mx * v.y + my * v.x);
-                        }
-                        outv[rr] = mk<T>(static_cast<T>(ox), static_cast<T>(oy));
-                    }
-                    if (dot) {
-                        for (uint32_t rr = 0; rr < R; ++rr)
-                            if (outok[rr]) acc_dot(accr, acci, B[outi[rr]], outv[rr]);
-                    } else {
-                        __syncthreads();
-                        for (uint32_t rr = 0; rr < R; ++rr) {
-                            const uint32_t e = r * per_round + rr * kBlock + tid;
-                            if (e >= tsize) continue;
-                            if (outok[rr]) S[outi[rr]] = outv[rr];
-                            else if (zero_unctrl) S[outi[rr]] = zero;
-                        }
-                        __syncthreads();
-                    }
-                }
-            }
-        } else if (kind == TOP_DIAG || kind == TOP_PCPHASE) {
-            const int k = static_cast<int>(op.k);
-            uint32_t idx_out = 0;
-            for (int j = 0; j < k; ++j)
-                if (op.lpos[j] < 0) idx_out |= static_cast<uint32_t>((base >> op.gpos[j]) & 1ull) << (k - 1 - j);
-            const double2 *D = a.coef + op.coef_off;
-            // DIAG k=0 holds one entry; only PCPHASE has a second one
-            const V d0 = ld_coef<T>(D, 0);
-            const V d1 = (kind == TOP_PCPHASE) ? ld_coef<T>(D, 1) : d0;
-            for (uint32_t l = tid; l < tsize; l += kBlock) {
-                const bool ok = (l & lcm) == lcv;
-                uint32_t idx = idx_out;
-                for (int j = 0; j < k; ++j)
-                    if (op.lpos[j] >= 0) idx |= ((l >> op.lpos[j]) & 1u) << (k - 1 - j);
-                V d;
-                if (kind == TOP_PCPHASE) d = (idx < op.pc_dim) ? d0 : d1;
-                else d = (k == 0) ? d0 : ld_coef<T>(D, idx);
-                if (dot) {
-                    if (ok) acc_dot(accr, acci, B[l], cmul(d, A[l]));
-                } else {
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (!((smask >> s) & 1u)) continue;
-                        V *S = lds + s * tsize;
-                        if (ok) S[l] = cmul(d, S[l]);
-                        else if (zero_unctrl) S[l] = zero;
-                    }
-                }
-            }
-        } else if (kind == TOP_PARITY) {
-            const uint32_t pout = __popcll(base & op.omask) & 1u;
-            const V d0 = ld_coef<T>(a.coef, op.coef_off), d1 = ld_coef<T>(a.coef, op.coef_off + 1);
-            for (uint32_t l = tid; l < tsize; l += kBlock) {
-                const bool ok = (l & lcm) == lcv;
-                const uint32_t par = (__popc(l & op.lxmask) & 1u) ^ pout;
-                const V d = par ? d1 : d0;
-                if (dot) {
-                    if (ok) acc_dot(accr, acci, B[l], cmul(d, A[l]));
-                } else {
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (!((smask >> s) & 1u)) continue;
-                        V *S = lds + s * tsize;
-                        if (ok) S[l] = cmul(d, S[l]);
-                        else if (zero_unctrl) S[l] = zero;
-                    }
-                }
-            }
-        } else if (kind == TOP_PAULI) {
-            const uint32_t lx = op.lxmask, lz = op.lzmask;
-            const uint32_t sout = __popcll(base & op.omask) & 1u;
-            const V cp = ld_coef<T>(a.coef, op.coef_off);
-            const V cm = mk<T>(-cp.x, -cp.y);
-            if (dot) {
-                for (uint32_t l = tid; l < tsize; l += kBlock) {
-                    if ((l & lcm) != lcv) continue;
-                    const V f = ((__popc(l & lz) & 1u) ^ sout) ? cm : cp;
-                    acc_dot(accr, acci, B[l], cmul(f, A[l ^ lx]));
-                }
-            } else if (lx == 0) {
-                for (uint32_t l = tid; l < tsize; l += kBlock) {
-                    const bool ok = (l & lcm) == lcv;
-                    const V f = ((__popc(l & lz) & 1u) ^ sout) ? cm : cp;
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (!((smask >> s) & 1u)) continue;
-                        V *S = lds + s * tsize;
-                        if (ok) S[l] = cmul(f, S[l]);
-                        else if (zero_unctrl) S[l] = zero;
-                    }
-                }
-            } else {
-                const int hb = 31 - __clz(lx);
-                for (uint32_t p = tid; p < (tsize >> 1); p += kBlock) {
-                    const uint32_t i = insert_zero(p, hb), j = i ^ lx;
-                    const bool ok = (i & lcm) == lcv;
-                    const V fi = ((__popc(i & lz) & 1u) ^ sout) ? cm : cp;
-                    const V fj = ((__popc(j & lz) & 1u) ^ sout) ? cm : cp;
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (!((smask >> s) & 1u)) continue;
-                        V *S = lds + s * tsize;
-                        if (ok) {
-                            const V vi = S[i], vj = S[j];
-                            S[i] = cmul(fi, vj);
-                            S[j] = cmul(fj, vi);
-                        } else if (zero_unctrl) {
-                            S[i] = zero;
-                            S[j] = zero;
-                        }
-                    }
-                }
-            }
-        }
-
+        dev::lds_apply_op<T, NS, false>(op, lds, tsize, base, a.coef, accr, acci);
         if (dot) block_reduce_store(accr, acci, red, dst);
         __syncthreads();
     }

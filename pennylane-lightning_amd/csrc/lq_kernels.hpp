@@ -38,6 +38,25 @@ struct SlotRange {
 
 template <typename T> void launch_tile_pass(const TilePassArgs &a, size_t lds_bytes, hipStream_t s);
 
+// Register-resident tile pass (lq_kernels_reg.hip): kt = 8 + RB, RB in {3, 4},
+// one or two states.
+struct TileRegArgs {
+    void *sv[2];
+    const TileOp *ops;
+    const TileStage *stages;
+    const double2 *coef;
+    double2 *partials;        // [slot][tile]
+    uint32_t nstages;
+    uint32_t nslots;          // dot slots of the pass (<= kMaxPassDots)
+    uint32_t load_mask;
+    uint32_t store_mask;
+    uint32_t end_transpose;   // last stage's mapping differs from the HBM mapping
+    uint32_t ns;
+    uint64_t ntiles;
+    TileGeom g;
+};
+template <typename T> void launch_tile_reg(const TileRegArgs &a, size_t lds_bytes, hipStream_t s);
+
 // out[slot] = sum of partials (Neumaier-compensated, fixed order)
 void launch_reduce_slots(const double2 *partials, const SlotRange *ranges, uint32_t nslots,
                          double2 *out, hipStream_t s);

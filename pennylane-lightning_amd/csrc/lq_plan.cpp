@@ -279,6 +279,219 @@ bool try_merge_1q(std::vector<LOp> &queue, const LOp &op) {
     return false;
 }
 
+namespace {
+// Ops of one stage of a register-kernel pass.
+struct StagePlan {
+    bool reg = false;
+    uint32_t R = 0;       // tile-local register bits (register stages)
+    std::vector<int> ops; // indices into the program
+};
+
+// Two ops must keep their order when a bit they share is acted on
+// non-diagonally by one of them; read-only dots commute with each other.
+bool conflicts(const LOp &a, const LOp &b) {
+    if (a.is_dot() && b.is_dot()) return false;
+    const uint64_t na = a.nmask(), da = a.dmask() & ~na, nb = b.nmask(), db = b.dmask() & ~nb;
+    return ((na & (nb | db)) | (da & nb)) != 0;
+}
+
+bool reg_capable(const LOp &o) {
+    if (o.flags & TF_ZERO_UNCTRL) return false;
+    switch (o.kind) {
+    case TOP_DENSE1:
+    case TOP_DENSE2:
+    case TOP_PARITY:
+    case TOP_PCPHASE: return true;
+    case TOP_DIAG: return o.k <= 2;
+    default: return false;
+    }
+}
+
+// Split a pass's ops into register stages (each with at most rb register
+// bits among its non-diagonal targets) and LDS stages, respecting the order
+// constraints between ops. Register bits of a stage are padded with the
+// previous stage's bits so that unchanged mappings need no transpose.
+template <typename LocalF>
+std::vector<StagePlan> plan_stages(const std::vector<LOp> &ops, const std::vector<int> &sel, LocalF local, int rb,
+                                   int kt) {
+    const int m = static_cast<int>(sel.size());
+    std::vector<std::vector<int>> succ(m);
+    std::vector<int> indeg(m, 0);
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < j; ++i)
+            if (conflicts(ops[sel[i]], ops[sel[j]])) {
+                succ[i].push_back(j);
+                ++indeg[j];
+            }
+    std::vector<uint32_t> nl(m);
+    std::vector<char> rc(m);
+    for (int i = 0; i < m; ++i) {
+        nl[i] = local(ops[sel[i]].nmask());
+        rc[i] = reg_capable(ops[sel[i]]) ? 1 : 0;
+    }
+    std::set<int> ready;
+    for (int i = 0; i < m; ++i)
+        if (indeg[i] == 0) ready.insert(i);
+    int left = m;
+    auto take = [&](int i) {
+        ready.erase(i);
+        --left;
+        for (int s : succ[i])
+            if (--indeg[s] == 0) ready.insert(s);
+    };
+    uint32_t prevR = ((1u << kt) - 1u) & ~0xFFu; // the load mapping's register bits
+    std::vector<StagePlan> out;
+    while (left > 0) {
+        StagePlan sp;
+        uint32_t U = 0;
+        while (true) {
+            int pick = -1;
+            for (int i : ready)
+                if (rc[i] && (nl[i] & ~U) == 0) {
+                    pick = i;
+                    break;
+                }
+            if (pick < 0)
+                for (int i : ready)
+                    if (rc[i] && popc(U | nl[i]) <= rb) {
+                        pick = i;
+                        break;
+                    }
+            if (pick < 0) break;
+            U |= nl[pick];
+            sp.ops.push_back(sel[pick]);
+            take(pick);
+        }
+        if (!sp.ops.empty()) {
+            sp.reg = true;
+            uint32_t R = U;
+            for (int b = 0; b < kt && popc(R) < rb; ++b)
+                if ((prevR >> b) & 1u) R |= 1u << b;
+            for (int b = kt - 1; b >= 0 && popc(R) < rb; --b) R |= 1u << b;
+            sp.R = R;
+            prevR = R;
+        } else {
+            while (true) {
+                int pick = -1;
+                for (int i : ready)
+                    if (!rc[i]) {
+                        pick = i;
+                        break;
+                    }
+                if (pick < 0) break;
+                sp.ops.push_back(sel[pick]);
+                take(pick);
+            }
+            LQ_ABORT_IF(sp.ops.empty(), "stage planning made no progress");
+        }
+        out.push_back(std::move(sp));
+    }
+    return out;
+}
+
+// Targets as register-index bits, controls and diagonal bits as global masks;
+// a diagonal +-1 table becomes a sign flip.
+void to_register_form(TileOp &t, const LOp &o, const TileStage &st, const int *pos, int rb) {
+    auto ridx = [&](int lb) -> int {
+        for (int i = 0; i < rb; ++i)
+            if (st.rbits[i] == lb) return i;
+        LQ_ABORT("register bit not found");
+    };
+    t.gcmask = o.cmask;
+    t.gcval = o.cval & o.cmask;
+    if (o.kind == TOP_DENSE1) {
+        t.lpos[0] = ridx(pos[o.bits[0]]);
+    } else if (o.kind == TOP_DENSE2) {
+        t.lpos[0] = ridx(pos[o.bits[0]]);
+        t.lpos[1] = ridx(pos[o.bits[1]]);
+    } else if (o.kind == TOP_PARITY) {
+        t.gpmask = o.pmask;
+    } else if (o.kind == TOP_DIAG && !o.is_dot()) {
+        const cd one(1.0), mone(-1.0);
+        if (o.k == 1 && o.coef[0] == one && o.coef[1] == mone) {
+            t.kind = TOP_SIGN;
+            t.gcmask |= 1ull << o.bits[0];
+            t.gcval |= 1ull << o.bits[0];
+        } else if (o.k == 1 && o.coef[0] == mone && o.coef[1] == one) {
+            t.kind = TOP_SIGN;
+            t.gcmask |= 1ull << o.bits[0];
+        } else if (o.k == 0 && o.coef[0] == mone) {
+            t.kind = TOP_SIGN;
+        }
+    }
+}
+
+// Merge runs of CZ sign flips into popcount-parity ops and set how each
+// stage's register mapping is reached.
+void finish_reg_pass(PassPlan &p, int kt) {
+    std::vector<TileOp> merged;
+    for (auto &st : p.stages) {
+        const uint32_t begin = static_cast<uint32_t>(merged.size());
+        TileOp cur{};
+        bool open = false;
+        auto flush = [&]() {
+            if (open) merged.push_back(cur);
+            open = false;
+        };
+        for (uint32_t i = st.op_begin; i < st.op_end; ++i) {
+            const TileOp &t = p.ops[i];
+            const bool cz = st.kind == STAGE_REG && t.kind == TOP_SIGN && !(t.flags & TF_DOT) &&
+                            popc(t.gcmask) == 2 && t.gcval == t.gcmask;
+            if (!cz) {
+                flush();
+                merged.push_back(t);
+                continue;
+            }
+            const int lo = __builtin_ctzll(t.gcmask), hi = 63 - __builtin_clzll(t.gcmask);
+            const uint32_t d = static_cast<uint32_t>(hi - lo);
+            int slot = -1;
+            if (open && cur.smask == t.smask) {
+                for (uint32_t z = 0; z < cur.nz; ++z)
+                    if (cur.sd[z] == d) slot = static_cast<int>(z);
+                if (slot < 0 && cur.nz < 4) slot = static_cast<int>(cur.nz++);
+            }
+            if (slot < 0) {
+                flush();
+                cur = t;
+                cur.kind = TOP_CZSET;
+                cur.gcmask = cur.gcval = 0;
+                cur.nz = 1;
+                cur.sd[0] = d;
+                cur.sm[0] = 0;
+                for (int z = 1; z < 4; ++z) {
+                    cur.sd[z] = 0;
+                    cur.sm[z] = 0;
+                }
+                open = true;
+                slot = 0;
+            }
+            cur.sd[slot] = d;
+            cur.sm[slot] |= 1ull << lo;
+        }
+        flush();
+        st.op_begin = begin;
+        st.op_end = static_cast<uint32_t>(merged.size());
+    }
+    p.ops.swap(merged);
+
+    const uint32_t m0 = ((1u << kt) - 1u) & ~0xFFu;
+    uint32_t prevR = m0;
+    bool prev_lds = false;
+    for (auto &st : p.stages) {
+        if (st.kind != STAGE_REG) {
+            prev_lds = true;
+            continue;
+        }
+        uint32_t R = 0;
+        for (int i = 0; i < kt - 8; ++i) R |= 1u << st.rbits[i];
+        st.entry = prev_lds ? ENTRY_FROM_LDS : (R == prevR ? ENTRY_NONE : ENTRY_TRANSPOSE);
+        prevR = R;
+        prev_lds = false;
+    }
+    p.end_transpose = (!prev_lds && prevR != m0) ? 1u : 0u;
+}
+} // namespace
+
 TileGeom make_geom(int n, const std::vector<int> &tbits) {
     TileGeom g{};
     g.n = n;
@@ -364,16 +577,18 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
         for (int attempt = 0; attempt < 2 && sel.empty(); ++attempt) {
             T = attempt == 0 ? ((low >= 64) ? ~0ull : ((1ull << low) - 1)) : 0ull;
             double cost = 0.0;
+            int dots = 0;
             while (true) {
                 int pick = -1;
+                const bool dots_full = dots >= cfg.max_dots;
                 for (int i : ready)
-                    if ((nm[i] & ~T) == 0) {
+                    if ((nm[i] & ~T) == 0 && !(dots_full && ops[i].is_dot())) {
                         pick = i;
                         break;
                     }
                 if (pick < 0)
                     for (int i : ready)
-                        if (popc(T | nm[i]) <= ktmax) {
+                        if (popc(T | nm[i]) <= ktmax && !(dots_full && ops[i].is_dot())) {
                             pick = i;
                             break;
                         }
@@ -381,6 +596,7 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
                 if (!sel.empty() && cost + ops[pick].cost() > cfg.budget) break;
                 T |= nm[pick];
                 cost += ops[pick].cost();
+                dots += ops[pick].is_dot() ? 1 : 0;
                 sel.push_back(pick);
                 ready.erase(pick);
                 --remaining;
@@ -406,8 +622,41 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
                 if (((m >> b) & 1ull) && pos[b] >= 0) r |= 1u << pos[b];
             return r;
         };
-        for (int idx : sel) {
+        const int kt = static_cast<int>(p.tbits.size());
+        p.reg = cfg.reg_rb > 0 && kt == 8 + cfg.reg_rb && cfg.ns <= 2;
+        std::vector<int> order = sel;
+        std::vector<StagePlan> stages;
+        if (p.reg) {
+            stages = plan_stages(ops, sel, local, cfg.reg_rb, kt);
+            order.clear();
+            for (const auto &st : stages) order.insert(order.end(), st.ops.begin(), st.ops.end());
+        }
+        std::vector<int> op_stage;
+        for (size_t si = 0; si < stages.size(); ++si)
+            for (size_t k2 = 0; k2 < stages[si].ops.size(); ++k2) op_stage.push_back(static_cast<int>(si));
+        size_t stage_cursor = 0;
+        for (size_t oi = 0; oi < order.size(); ++oi) {
+            const int idx = order[oi];
             const LOp &o = ops[idx];
+            if (p.reg) {
+                const int si = op_stage[oi];
+                if (p.stages.size() <= static_cast<size_t>(si)) {
+                    // open the stage
+                    TileStage ts{};
+                    ts.kind = stages[si].reg ? STAGE_REG : STAGE_LDS;
+                    ts.op_begin = ts.op_end = static_cast<uint32_t>(p.ops.size());
+                    int r = 0, h = 0;
+                    for (int b = 0; b < kt; ++b) {
+                        if ((stages[si].R >> b) & 1u) {
+                            if (r < kMaxRegBits) ts.rbits[r++] = b;
+                        } else if (h < 8) {
+                            ts.hbits[h++] = b;
+                        }
+                    }
+                    p.stages.push_back(ts);
+                    stage_cursor = static_cast<size_t>(si);
+                }
+            }
             TileOp t{};
             t.kind = o.kind;
             t.flags = o.flags;
@@ -443,8 +692,12 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
                 p.load_mask |= o.smask;
                 p.store_mask |= o.smask;
             }
+            if (p.reg && p.stages[stage_cursor].kind == STAGE_REG)
+                to_register_form(t, o, p.stages[stage_cursor], pos, cfg.reg_rb);
             p.ops.push_back(t);
+            if (p.reg) p.stages[stage_cursor].op_end = static_cast<uint32_t>(p.ops.size());
         }
+        if (p.reg) finish_reg_pass(p, kt);
         passes.push_back(std::move(p));
     }
     return passes;

@@ -64,6 +64,10 @@ struct PassPlan {
     std::vector<cd> coef;
     uint32_t load_mask = 0, store_mask = 0;
     std::vector<int32_t> slot_ids; // caller slot id of each local dot slot
+    // register-resident form (tile_reg_kernel): ops grouped into stages
+    bool reg = false;
+    std::vector<TileStage> stages;
+    uint32_t end_transpose = 0;
 };
 
 struct PlanConfig {
@@ -72,6 +76,8 @@ struct PlanConfig {
     int kt_max = 12;     // max tile bits
     int low_bits = 4;    // contiguous low bits always in the tile
     double budget = 24;  // max op cost per pass
+    int reg_rb = 0;      // > 0: emit register stages for tiles of 8 + reg_rb bits
+    int max_dots = 1 << 30; // dot slots per pass
 };
 
 std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig &cfg);

@@ -29,6 +29,26 @@ enum TileOpKind : uint32_t {
     TOP_PARITY = 5,  // coef[parity(index & mask)] (MultiRZ, Pauli-Z products)
     TOP_PCPHASE = 6, // local index < dim ? coef[0] : coef[1]  (PCPhase)
     TOP_PAULI = 7,   // Pauli string: (-i)^ny (-1)^popcnt(i & yzmask) a[i ^ xmask]
+    TOP_SIGN = 8,    // negate where (index & gcmask) == gcval  (Z, CZ, multi-controlled Z)
+    TOP_CZSET = 9,   // negate where sum_i popcnt(index & (index >> sd[i]) & sm[i]) is odd
+};
+
+// Register-resident tile programs (tile_reg_kernel): a pass is a list of
+// stages; in a REG stage every thread holds 2^RB amplitudes per state whose
+// tile-local indices differ in the stage's register bits; non-diagonal
+// targets of the stage's ops are register bits. LDS stages run ops on the
+// LDS copy of the tile (ops the register path does not cover).
+enum TileStageKind : uint32_t { STAGE_REG = 1, STAGE_LDS = 2 };
+enum TileStageEntry : uint32_t { ENTRY_NONE = 0, ENTRY_TRANSPOSE = 1, ENTRY_FROM_LDS = 2 };
+constexpr int kMaxRegBits = 4;    // 16 amplitudes per state per thread
+constexpr int kMaxPassDots = 32;  // dot slots per register-kernel pass
+
+struct TileStage {
+    uint32_t kind;
+    uint32_t op_begin, op_end;
+    uint32_t entry;                 // how the stage's mapping is reached
+    int32_t rbits[kMaxRegBits];     // tile-local bit of register index bit i
+    int32_t hbits[8];               // tile-local bit of thread index bit j
 };
 
 enum TileOpFlags : uint32_t {
@@ -53,6 +73,13 @@ struct alignas(16) TileOp {
     uint64_t omask;   // PARITY/PAULI: outside bits contributing to the sign/parity
     uint64_t coef_off; // offset (complex entries) into the pass coefficient buffer
     uint64_t pc_dim;   // PCPHASE: threshold on the local index
+    uint64_t gcmask;   // controls on global bits (register stages)
+    uint64_t gcval;
+    uint64_t gpmask;   // PARITY: global mask (register stages)
+    uint64_t sm[4];    // CZSET: bit a of sm[i] pairs bits a and a + sd[i]
+    uint32_t sd[4];
+    uint32_t nz;       // CZSET: number of (sd, sm) terms
+    uint32_t pad2;
     int32_t lpos[kMaxTileOpTargets]; // tile-local position of target j (-1: outside)
     int32_t gpos[kMaxTileOpTargets]; // global bit of target j (target 0 = local MSB)
 };

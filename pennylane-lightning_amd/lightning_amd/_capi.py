@@ -6,6 +6,7 @@ ImportError. There is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 
@@ -69,12 +70,19 @@ _SIGS = {
     "lq_sv_profile": (C.c_int, [_P, C.c_int]),
     "lq_sv_profile_read": (C.c_int, [_P, C.c_int, _U64P, _DP, _DP, _DP]),
     "lq_sv_stream": (C.c_int, [_P, C.POINTER(_P)]),
+    "lq_jit_wait": (C.c_int, []),
+    "lq_jit_shutdown": (C.c_int, []),
+    "lq_jit_stats": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "lq_sv_pass_counts": (C.c_int, [_P, C.c_int, _U64P, _U64P]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
     _f.restype = _res
     _f.argtypes = _args
+
+# background pass compiles must not be in flight while the process tears down
+atexit.register(lib.lq_jit_shutdown)
 
 
 class LightningException(RuntimeError):

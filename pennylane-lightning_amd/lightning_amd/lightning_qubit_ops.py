@@ -58,6 +58,18 @@ def runtime_info():
     return {"devices": n.value}
 
 
+def jit_wait():
+    """Block until every queued pass compile has finished."""
+    check(lib.lq_jit_wait())
+
+
+def jit_stats() -> dict:
+    """Specialised kernels loaded, compiles failed, compiles pending."""
+    r, f, p = C.c_int(), C.c_int(), C.c_int()
+    check(lib.lq_jit_stats(C.byref(r), C.byref(f), C.byref(p)))
+    return {"ready": r.value, "failed": f.value, "pending": p.value}
+
+
 def _is_bool(x) -> bool:
     return isinstance(x, (bool, np.bool_))
 
@@ -152,6 +164,12 @@ class _StateVectorBase:
 
     def release_workspace(self):
         check(lib.lq_sv_release_workspace(self._h))
+
+    def pass_counts(self, reset: bool = True) -> dict:
+        """Tile passes run as specialised kernels and as generic tile kernels."""
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib.lq_sv_pass_counts(self._h, int(bool(reset)), C.byref(a), C.byref(b)))
+        return {"specialised": a.value, "generic": b.value}
 
     def copy_from(self, other):
         check(lib.lq_sv_copy(self._h, other._h))

@@ -1,0 +1,128 @@
+"""Specialised pass kernels vs the oracle.
+
+Register passes of states with >= 16 qubits are compiled into straight-line
+kernels in the background; until a kernel is ready the pass runs in the
+generic tile kernel. Each case runs a circuit once (generic kernels, compiles
+queued), waits for the compiles, runs it again (specialised kernels) and
+checks both results against the oracle, and that the second run did use
+specialised kernels.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import TOL_C128, TOL_C64, ham_to_oracle, oracle_run, to_oracle_ops
+from oracle import lq_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L = pytest.importorskip("lightning_amd.lightning_qubit_ops")
+W = pytest.importorskip("lightning_amd.workloads")
+
+SV = {128: L.StateVectorC128, 64: L.StateVectorC64}
+TOL = {128: TOL_C128, 64: TOL_C64 * 10}
+
+# gates whose lowered forms cover every op kind of the register passes:
+# dense 1q/2q, diagonal k <= 2, parity (MultiRZ), PCPhase, sign flips and CZ runs
+GATES = ["RX", "RY", "RZ", "Rot", "Hadamard", "PauliX", "PauliY", "PauliZ", "S", "T", "SX", "PhaseShift",
+         "CNOT", "CY", "CZ", "SWAP", "IsingXX", "IsingYY", "IsingZZ", "IsingXY", "CRX", "CRY", "CRZ", "CRot",
+         "ControlledPhaseShift", "SingleExcitation", "MultiRZ", "PCPhase", "GlobalPhase"]
+
+
+def random_ops(n, count, seed):
+    rng = np.random.default_rng(seed)
+    ops = []
+    for _ in range(count):
+        name = GATES[int(rng.integers(len(GATES)))]
+        nw = O.GATE_WIRES[name] if O.GATE_WIRES[name] > 0 else 2
+        params = list(rng.uniform(-math.pi, math.pi, size=O.GATE_NPARAMS[name]))
+        if name == "PCPhase":
+            params[1] = float(rng.integers(0, 1 << nw))
+        wires = [int(w) for w in rng.permutation(n)[:nw]]
+        ctrls, cvals = [], []
+        if name in O.CONTROLLED_GATES and name != "GlobalPhase" and rng.random() < 0.25:
+            rest = [w for w in range(n) if w not in wires]
+            ctrls = [int(rest[int(rng.integers(len(rest)))])]
+            cvals = [bool(rng.integers(0, 2))]
+        ops.append(W.CircuitOp(name, wires, params, bool(rng.integers(0, 2)), ctrls=ctrls, cvals=cvals))
+    # CZ ladders: runs of sign flips merged into parity ops
+    for w in range(n - 1):
+        ops.append(W.CircuitOp("CZ", [w, w + 1]))
+    ops += [W.CircuitOp("RY", [w], [0.1 * (w + 1)]) for w in range(n)]
+    for w in range(0, n - 2, 2):
+        ops.append(W.CircuitOp("CZ", [w, w + 2]))
+    return ops
+
+
+def run_twice(make, check):
+    """make() runs the workload and returns (result, handle). The first run
+    queues the compiles; the second runs after they finish."""
+    r1, h1 = make()
+    h1.pass_counts(reset=True)
+    L.jit_wait()
+    r2, h2 = make()
+    counts = h2.pass_counts(reset=True)
+    check(r1)
+    check(r2)
+    assert counts["specialised"] > 0, counts
+    assert L.jit_stats()["failed"] == 0
+
+
+@pytest.mark.parametrize("bits", [128, 64])
+@pytest.mark.parametrize("n,seed", [(16, 3), (18, 5)])
+def test_jit_random_circuit(bits, n, seed):
+    ops = random_ops(n, 160, seed)
+    exp = oracle_run(n, ops)
+
+    def make():
+        sv = SV[bits](n)
+        W.apply_circuit(sv, ops)
+        return sv.state(), sv
+
+    run_twice(make, lambda st: np.testing.assert_allclose(st, exp, atol=TOL[bits]))
+
+
+@pytest.mark.parametrize("bits", [128, 64])
+def test_jit_expvals(bits):
+    n = 17
+    ops = W.hea_rot_cz(n, 3, seed=11)
+    st = O.apply_ops(O.zero_state(n), n, to_oracle_ops(ops))
+    names = [("PauliZ", [0]), ("PauliX", [16]), ("PauliY", [8]), ("Hadamard", [3])]
+    exp = [O.expval_named(st, n, nm, w) for nm, w in names]
+    meas = {128: L.MeasurementsC128, 64: L.MeasurementsC64}[bits]
+
+    def make():
+        sv = SV[bits](n)
+        W.apply_circuit(sv, ops)
+        m = meas(sv)
+        return [m.expval(nm, w) for nm, w in names], sv
+
+    run_twice(make, lambda ev: np.testing.assert_allclose(ev, exp, atol=TOL[bits] * 10))
+
+
+@pytest.mark.parametrize("bits", [128, 64])
+def test_jit_adjoint(bits):
+    n = 16
+    ops = W.strongly_entangling(n, 1, seed=8)
+    ops += [W.CircuitOp("CRY", [2, 9], [0.3]), W.CircuitOp("IsingZZ", [0, 15], [0.8], inverse=True),
+            W.CircuitOp("MultiRZ", [1, 5, 14], [0.6]), W.CircuitOp("PhaseShift", [7], [1.1])]
+    coeffs, words = W.random_pauli_hamiltonian(n, 12, seed=2, law=(0.5, 0.2, 0.15, 0.15))
+    npar = sum(1 for op in ops if op.params)
+    tp = list(range(0, npar, 2))
+    st = oracle_run(n, ops)
+    oobs = [ham_to_oracle(coeffs, words), ("N", "PauliZ", 4)]
+    jexp = O.adjoint_jacobian(st, n, to_oracle_ops(ops), oobs, tp)
+    sfx = f"C{bits}"
+    tol = 1e-10 if bits == 128 else 5e-4
+
+    def make():
+        sv = SV[bits](n)
+        W.apply_circuit(sv, ops)
+        H = W.hamiltonian_obs(coeffs, words, bits)
+        Z = getattr(L, "NamedObs" + sfx)("PauliZ", [4])
+        adj = getattr(L.algorithms, "AdjointJacobian" + sfx)()
+        jac = adj(sv, [H, Z], W.to_ops(ops, bits), tp)
+        return np.asarray(jac).reshape(2, -1), sv
+
+    run_twice(make, lambda j: np.testing.assert_allclose(j, jexp, atol=tol))

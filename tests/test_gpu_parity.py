@@ -38,11 +38,11 @@ def make_sv(bits, n, psi):
     return sv
 
 
+@pytest.mark.parametrize("n", [7, 12])
 @pytest.mark.parametrize("bits", [128, 64])
 @pytest.mark.parametrize("name", ALL_GATES)
-def test_named_gates(bits, name):
-    rng = np.random.default_rng(abs(hash((name, bits))) % 2**32)
-    n = 7
+def test_named_gates(bits, name, n):
+    rng = np.random.default_rng(abs(hash((name, bits, n))) % 2**32)
     for inverse in (False, True):
         nw, params = gate_args(name, rng)
         wires = [int(w) for w in rng.permutation(n)[:nw]]
@@ -53,10 +53,10 @@ def test_named_gates(bits, name):
         np.testing.assert_allclose(sv.state(), exp, atol=TOL[bits], err_msg=f"{name} inv={inverse}")
 
 
+@pytest.mark.parametrize("n", [8, 13])
 @pytest.mark.parametrize("name", sorted(O.CONTROLLED_GATES))
-def test_controlled_gates(name):
-    rng = np.random.default_rng(abs(hash(name)) % 2**32)
-    n = 8
+def test_controlled_gates(name, n):
+    rng = np.random.default_rng(abs(hash((name, n))) % 2**32)
     for nc in (1, 2, 3):
         nw, params = gate_args(name, rng, k_multi=2)
         perm = [int(w) for w in rng.permutation(n)]
@@ -73,7 +73,7 @@ def test_controlled_gates(name):
 @pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
 def test_matrices(k):
     rng = np.random.default_rng(k)
-    n = 9
+    n = 12
     for nc in (0, 1, 2):
         perm = [int(w) for w in rng.permutation(n)]
         wires, ctrls = perm[:k], perm[k:k + nc]
@@ -207,7 +207,7 @@ def test_state_prep_and_misc():
                                   "DoubleExcitationMinus", "DoubleExcitationPlus", "MultiRZ", "GlobalPhase"])
 def test_generators(name):
     rng = np.random.default_rng(abs(hash(name)) % 2**32)
-    n = 8
+    n = 12
     nw = O.GATE_WIRES[name] if O.GATE_WIRES[name] > 0 else 3
     for nc in (0, 1):
         if nc and name in ("CRX", "CRY", "CRZ", "ControlledPhaseShift"):
