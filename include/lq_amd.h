@@ -131,6 +131,14 @@ int lq_sv_apply_ops(lq_sv_t sv, lq_ops_t ops, int adjoint);
  * if not NULL, receives <obs> on the final state. sv is not modified. */
 int lq_adjoint_jacobian(lq_sv_t sv, lq_ops_t ops, const lq_obs_t *obs, uint32_t nobs, const uint64_t *trainable,
                         uint32_t ntp, int apply_operations, double *jac, double *expvals);
+/* ---- Vector-Jacobian product: VectorJacobianProduct<SV>::operator()
+ * (lq/algorithms/VectorJacobianProduct.hpp:43-160; binding
+ * lq/bindings/LQubitBindings.hpp:412-457). dy is a host array of len (= 2^n)
+ * amplitudes in the handle's precision (the state cotangent); out receives
+ * ntp complex values as (re, im) doubles, w_t = <dy | d psi / d theta_t>.
+ * sv is not modified. */
+int lq_vector_jacobian_product(lq_sv_t sv, lq_ops_t ops, const void *dy, uint64_t len, const uint64_t *trainable,
+                               uint32_t ntp, int apply_operations, double *out);
 /* ---- Instrumentation (no reference counterpart) ----------------------------
  * When enabled, every tile-pass kernel launch on the handle's stream is
  * bracketed by HIP events; profile_read totals the measured kernel time, the

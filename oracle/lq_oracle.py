@@ -550,6 +550,44 @@ def adjoint_jacobian(final_state: np.ndarray, n: int, ops: Sequence[Op],
     return jac
 
 
+def vector_jacobian_product(final_state: np.ndarray, n: int, ops: Sequence[Op], dy: np.ndarray,
+                            trainable: Sequence[int]) -> np.ndarray:
+    """VectorJacobianProduct::operator() with apply_operations=false
+    (lq/algorithms/VectorJacobianProduct.hpp:84-157).
+
+    mu starts as the state cotangent dy; for each op from the last, if it is
+    the next trainable parametric op, w[t] = i s <G mu | lambda> (:128-146,
+    s negated for inverse ops), then lambda and mu get U^dagger (:152-155).
+    Returns len(trainable) complex values.
+    """
+    tp = sorted(int(t) for t in trainable)
+    w = np.zeros(len(tp), dtype=np.complex128)
+    if not tp:
+        return w
+    lam = final_state.astype(np.complex128).copy()
+    mu = np.asarray(dy, dtype=np.complex128).copy()
+    cur = sum(1 for op in ops if len(op.params) > 0) - 1
+    tnum = len(tp) - 1
+    for op in reversed(ops):
+        if len(op.params) > 1:
+            raise RuntimeError("The operation is not supported using the adjoint differentiation method")
+        if op.name in ("StatePrep", "BasisState"):
+            continue
+        if tnum < 0:
+            break
+        if len(op.params) > 0:
+            if cur == tp[tnum]:
+                mu_d = mu.copy()
+                s = apply_generator(mu_d, n, op.name, op.wires, op.ctrls, op.cvals)
+                s = s * (-1.0 if op.inverse else 1.0)
+                w[tnum] = 1j * s * np.vdot(mu_d, lam)
+                tnum -= 1
+            cur -= 1
+        _apply_op_adj(lam, n, op)
+        _apply_op_adj(mu, n, op)
+    return w
+
+
 # ---------------------------------------------------------------------------
 # Seeded circuits of the measured workloads (identical ops lists are fed to the
 # reference, the oracle and the engine)

@@ -205,6 +205,48 @@ class RefStateVector:
             P(jac, C.c_double)))
         return jac[: len(observables) * len(trainable)].reshape(len(observables), len(trainable))
 
+    def vector_jacobian_product(self, ops, dy, trainable) -> np.ndarray:
+        """VectorJacobianProduct on this (final) state; returns len(trainable) complex."""
+        a = _pack_ops(ops)
+        names = (C.c_char_p * max(len(ops), 1))(*[op.name.encode() for op in ops])
+        P = lambda arr, t: arr.ctypes.data_as(C.POINTER(t))
+        d = np.ascontiguousarray(np.asarray(dy).reshape(-1),
+                                 dtype=np.complex128 if self.prec == 1 else np.complex64)
+        tp = np.asarray(list(trainable) + [0], dtype=np.uint64)
+        out = np.zeros(2 * max(len(trainable), 1), dtype=np.float64)
+        _check(lib().ref_vjp(
+            self._h(), self.prec, C.c_uint32(len(ops)), names,
+            P(a["poff"], C.c_uint32), P(a["params"], C.c_double), P(a["woff"], C.c_uint32),
+            P(a["wires"], C.c_uint32), P(a["invs"], C.c_uint8), P(a["moff"], C.c_uint32),
+            P(a["mats"], C.c_double), P(a["coff"], C.c_uint32), P(a["ctrls"], C.c_uint32),
+            P(a["cvals"], C.c_uint8), d.ctypes.data_as(C.c_void_p), C.c_uint64(d.size),
+            P(tp, C.c_uint64), C.c_uint32(len(trainable)), P(out, C.c_double)))
+        return out[: 2 * len(trainable)].view(np.complex128).copy()
+
+
+def _pack_ops(ops):
+    poff, woff, moff, coff = [0], [0], [0], [0]
+    params, wires, mats, ctrls, cvals, invs = [], [], [], [], [], []
+    for op in ops:
+        params += [float(x) for x in op.params]
+        poff.append(len(params))
+        wires += list(op.wires)
+        woff.append(len(wires))
+        if op.matrix is not None:
+            m = np.asarray(op.matrix, dtype=np.complex128).reshape(-1)
+            mats += [x for z in m for x in (z.real, z.imag)]
+        moff.append(len(mats) // 2)
+        ctrls += list(op.ctrls)
+        cvals += [int(bool(v)) for v in (op.cvals if len(op.cvals) else [1] * len(op.ctrls))]
+        coff.append(len(ctrls))
+        invs.append(int(bool(op.inverse)))
+    return dict(
+        poff=np.asarray(poff, np.uint32), params=np.asarray(params + [0.0], np.float64),
+        woff=np.asarray(woff, np.uint32), wires=np.asarray(wires + [0], np.uint32),
+        invs=np.asarray(invs + [0], np.uint8), moff=np.asarray(moff, np.uint32),
+        mats=np.asarray(mats + [0.0], np.float64), coff=np.asarray(coff, np.uint32),
+        ctrls=np.asarray(ctrls + [0], np.uint32), cvals=np.asarray(cvals + [0], np.uint8))
+
 
 def _known():
     from .lq_oracle import GATE_NPARAMS

@@ -28,6 +28,7 @@
 #include "MeasurementsLQubit.hpp"
 #include "ObservablesLQubit.hpp"
 #include "StateVectorLQubitManaged.hpp"
+#include "VectorJacobianProduct.hpp"
 
 using namespace Pennylane::LightningQubit;
 using namespace Pennylane::LightningQubit::Measures;
@@ -321,6 +322,31 @@ int ref_adjoint(void *h, int prec, uint32_t nops, const char *const *names,
         AdjointJacobian<SV<T>> adj;
         adj.adjointJacobian(std::span<T>{j}, jd, s, false);
         for (std::size_t i = 0; i < j.size(); i++) jac[i] = j[i];
+    }))
+}
+
+// VectorJacobianProduct on the handle's (final) state; dy: len complex values
+// of the handle's precision; out: ntp complex values as (re, im) doubles.
+int ref_vjp(void *h, int prec, uint32_t nops, const char *const *names,
+            const uint32_t *param_off, const double *params,
+            const uint32_t *wire_off, const uint32_t *wires,
+            const uint8_t *inverses, const uint32_t *mat_off,
+            const double *mats, const uint32_t *ctrl_off,
+            const uint32_t *ctrls, const uint8_t *ctrl_vals,
+            const void *dy, uint64_t len, const uint64_t *tp, uint32_t ntp, double *out) {
+    GUARD(DISPATCH(prec, {
+        auto ops = make_ops<T>(nops, names, param_off, params, wire_off, wires, inverses,
+                               mat_off, mats, ctrl_off, ctrls, ctrl_vals);
+        std::vector<std::size_t> tpv(tp, tp + ntp);
+        auto &s = sv<T>(h);
+        JacobianData<SV<T>> jd(ops.getTotalNumParams(), s.getLength(), s.getData(), {}, ops, tpv);
+        std::vector<std::complex<T>> w(ntp);
+        Pennylane::LightningQubit::Algorithms::VectorJacobianProduct<SV<T>> vjp;
+        vjp(std::span{w}, jd, std::span<const std::complex<T>>{static_cast<const std::complex<T> *>(dy), len}, false);
+        for (std::size_t i = 0; i < w.size(); i++) {
+            out[2 * i] = w[i].real();
+            out[2 * i + 1] = w[i].imag();
+        }
     }))
 }
 

@@ -296,6 +296,21 @@ int lq_adjoint_jacobian(lq_sv_t sv, lq_ops_t ops, const lq_obs_t *obs, uint32_t 
     });
 }
 
+int lq_vector_jacobian_product(lq_sv_t sv, lq_ops_t ops, const void *dy, uint64_t len, const uint64_t *trainable,
+                               uint32_t ntp, int apply_operations, double *out) {
+    return guard([&] {
+        LQ_ABORT_IF(!ops, "null ops");
+        LQ_ABORT_IF(!dy, "null cotangent");
+        std::vector<uint64_t> tp(trainable, trainable + ntp);
+        std::vector<cd> w(ntp);
+        vector_jacobian_product(S(sv), ops->list, dy, len, tp, apply_operations != 0, w.data());
+        for (uint32_t t = 0; t < ntp; ++t) {
+            out[2 * t] = w[t].real();
+            out[2 * t + 1] = w[t].imag();
+        }
+    });
+}
+
 int lq_sv_profile(lq_sv_t sv, int enable) {
     return guard([&] { S(sv).profile(enable != 0); });
 }

@@ -891,6 +891,97 @@ double StateVector::expval_obs(const Obs &o) {
 // ---------------------------------------------------------------------------
 // Adjoint Jacobian (lq/algorithms/AdjointJacobianLQubit.hpp:225-491)
 // ---------------------------------------------------------------------------
+namespace {
+
+// Lowered op of an OpsList entry, as applied forward (inverse flag honoured)
+// or, with `dagger`, as its adjoint U^dagger
+// (core/algorithms/AdjointJacobianBase.hpp:108-127).
+LOp lower_record(int n, const OpRec &op, bool dagger) {
+    const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
+    const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
+    const bool inv = op.inverse != dagger;
+    if (op.named)
+        return lower_gate(n, op.gid, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
+                          static_cast<int>(op.ctrls.size()), inv, op.params.data());
+    // the reference resolves a controlled op by name only
+    // (lq/StateVectorLQubit.hpp:457-482, DynamicDispatcher.hpp:385-388)
+    LQ_ABORT_IF(!op.ctrls.empty(), (std::string("Controlled operation does not exist for ") + op.name).c_str());
+    std::vector<cd> mm = op.matrix;
+    const size_t d = static_cast<size_t>(1) << op.wires.size();
+    LQ_ABORT_IF(mm.size() != d * d, (std::string("Operation does not exist for ") + op.name).c_str());
+    if (inv) {
+        std::vector<cd> t(d * d);
+        for (size_t r = 0; r < d; ++r)
+            for (size_t c = 0; c < d; ++c) t[c * d + r] = std::conj(mm[r * d + c]);
+        mm.swap(t);
+    }
+    std::vector<int> tb;
+    for (auto w : op.wires) tb.push_back(phys_bit(n, w));
+    return lower_matrix(std::move(mm), tb, 0, 0, false);
+}
+
+void apply_forward(StateVector &sv, const OpsList &ops, void *state) {
+    std::vector<LOp> fwd;
+    for (const auto &op : ops.ops) {
+        if (op.named && op.gid == GateId::Identity) continue;
+        fwd.push_back(lower_record(sv.n(), op, false));
+    }
+    sv.run(fwd, {state}, 0);
+}
+
+// The backward sweep (lq/algorithms/AdjointJacobianLQubit.hpp:225-319 and
+// lq/algorithms/VectorJacobianProduct.hpp:84-157) as one program on the
+// states {lambda, b_1 .. b_m}: for each trainable op, from the last, the dots
+// <b_j | G lambda> into slot slot0 + t*m + j (t = position in the sorted
+// trainable list), then U^dagger on every state. Ops before the first
+// trainable parameter change no result and are left out. Returns the
+// generator scales, negated for inverse ops.
+std::vector<double> backward_sweep(int n, const OpsList &ops, const std::vector<uint64_t> &tp, size_t m, int slot0,
+                                   std::vector<LOp> &prog) {
+    const size_t ntp = tp.size();
+    const uint32_t all = (1u << (m + 1)) - 1u;
+    std::vector<double> scales(ntp, 0.0);
+    size_t tp_pos = ntp; // tp[tp_pos-1] is the next trainable index going backwards
+    int64_t cur = static_cast<int64_t>(ops.num_par_ops) - 1;
+    int64_t tnum = static_cast<int64_t>(ntp) - 1;
+    size_t last_dot = prog.size();
+    for (int64_t k = static_cast<int64_t>(ops.ops.size()) - 1; k >= 0; --k) {
+        const OpRec &op = ops.ops[static_cast<size_t>(k)];
+        LQ_ABORT_IF(op.params.size() > 1, "The operation is not supported using the adjoint differentiation method");
+        if (op.name == "StatePrep" || op.name == "BasisState") continue;
+        if (tp_pos == 0) break;
+        if (!op.params.empty()) {
+            if (cur == static_cast<int64_t>(tp[tp_pos - 1])) {
+                LQ_ABORT_IF(!op.named || !gate_info(op.gid).has_generator,
+                            (std::string("The operation is not supported using the adjoint differentiation method: ") +
+                             op.name).c_str());
+                double s = 0.0;
+                const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
+                const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
+                LOp g = lower_generator(n, op.gid, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
+                                        static_cast<int>(op.ctrls.size()), &s);
+                scales[static_cast<size_t>(tnum)] = s * (op.inverse ? -1.0 : 1.0);
+                for (size_t j = 0; j < m; ++j)
+                    prog.push_back(make_dot(g, 0, 1 + static_cast<uint32_t>(j),
+                                            slot0 + static_cast<int>(tnum * static_cast<int64_t>(m) + j)));
+                last_dot = prog.size();
+                --tnum;
+                --tp_pos;
+            }
+            --cur;
+        }
+        if (op.named && op.gid == GateId::Identity) continue;
+        LOp u = lower_record(n, op, true);
+        u.smask = all;
+        prog.push_back(std::move(u));
+    }
+    // ops after the last dot change no result
+    prog.resize(last_dot);
+    return scales;
+}
+
+} // namespace
+
 void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std::shared_ptr<Obs>> &obs,
                       const std::vector<uint64_t> &tp_in, bool apply_operations, double *jac, double *expvals) {
     std::vector<uint64_t> tp = tp_in;
@@ -898,37 +989,11 @@ void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std
     const size_t ntp = tp.size(), nobs = obs.size();
     if (ntp == 0) return;
     LQ_ABORT_IF(nobs == 0, "no observables given");
-    const int n = sv.n();
     sv.flush();
-    // lambda, kept unchanged across observable chunks as workspace 0
+    // lambda, kept unchanged across observables as workspace 0
     void *lam0 = sv.workspace(0);
     hip_check(hipMemcpyAsync(lam0, sv.data(), sv.bytes(), hipMemcpyDeviceToDevice, sv.stream()), "memcpy");
-    if (apply_operations) {
-        std::vector<LOp> fwd;
-        for (const auto &op : ops.ops) {
-            if (op.name == "Identity") continue;
-            // reuse StateVector lowering through a temporary queue on lambda
-            const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
-            const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
-            if (op.named) {
-                fwd.push_back(lower_gate(n, op.gid, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
-                                         static_cast<int>(op.ctrls.size()), op.inverse, op.params.data()));
-            } else {
-                std::vector<cd> m = op.matrix;
-                const size_t d = static_cast<size_t>(1) << op.wires.size();
-                if (op.inverse) {
-                    std::vector<cd> t(d * d);
-                    for (size_t r = 0; r < d; ++r)
-                        for (size_t c = 0; c < d; ++c) t[c * d + r] = std::conj(m[r * d + c]);
-                    m.swap(t);
-                }
-                std::vector<int> tb;
-                for (auto w : op.wires) tb.push_back(phys_bit(n, w));
-                fwd.push_back(lower_matrix(std::move(m), tb, 0, 0, false));
-            }
-        }
-        sv.run(fwd, {lam0}, 0);
-    }
+    if (apply_operations) apply_forward(sv, ops, lam0);
     // one H|lambda> per sweep keeps two tiles in LDS; more observables re-run
     // the sweep from a saved lambda
     const size_t per_sweep = 1;
@@ -946,71 +1011,12 @@ void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std
             sv.apply_obs_to(*obs[o0 + j], lam, h);
             states.push_back(h);
         }
-        const uint32_t all = (1u << states.size()) - 1u;
         std::vector<LOp> prog;
-        std::vector<double> scales(ntp, 0.0);
-        int nslot = 0;
+        int exp_slots = 0;
         if (expvals) {
-            for (size_t j = 0; j < m; ++j) prog.push_back(identity_dot(0, 1 + static_cast<uint32_t>(j), nslot++));
+            for (size_t j = 0; j < m; ++j) prog.push_back(identity_dot(0, 1 + static_cast<uint32_t>(j), exp_slots++));
         }
-        const int exp_slots = nslot;
-        size_t tp_pos = ntp; // tp[tp_pos-1] is the next trainable index going backwards
-        int64_t cur = static_cast<int64_t>(ops.num_par_ops) - 1;
-        int64_t tnum = static_cast<int64_t>(ntp) - 1;
-        size_t last_dot = 0;
-        for (int64_t k = static_cast<int64_t>(ops.ops.size()) - 1; k >= 0; --k) {
-            const OpRec &op = ops.ops[static_cast<size_t>(k)];
-            LQ_ABORT_IF(op.params.size() > 1, "The operation is not supported using the adjoint differentiation method");
-            if (op.name == "StatePrep" || op.name == "BasisState") continue;
-            if (tp_pos == 0) break;
-            if (!op.params.empty()) {
-                if (cur == static_cast<int64_t>(tp[tp_pos - 1])) {
-                    LQ_ABORT_IF(!op.named || !gate_info(op.gid).has_generator,
-                                (std::string("The operation is not supported using the adjoint differentiation method: ") +
-                                 op.name).c_str());
-                    double s = 0.0;
-                    const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
-                    const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
-                    LOp g = lower_generator(n, op.gid, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
-                                            static_cast<int>(op.ctrls.size()), &s);
-                    scales[static_cast<size_t>(tnum)] = s * (op.inverse ? -1.0 : 1.0);
-                    for (size_t j = 0; j < m; ++j)
-                        prog.push_back(make_dot(g, 0, 1 + static_cast<uint32_t>(j),
-                                                exp_slots + static_cast<int>(tnum * static_cast<int64_t>(m) + j)));
-                    last_dot = prog.size();
-                    --tnum;
-                    --tp_pos;
-                }
-                --cur;
-            }
-            // U^dagger on lambda and every H lambda
-            LOp u;
-            const uint32_t *cw = op.ctrls.empty() ? nullptr : op.ctrls.data();
-            const uint8_t *cv = op.cvals.empty() ? nullptr : op.cvals.data();
-            if (op.named) {
-                if (op.gid == GateId::Identity) continue;
-                u = lower_gate(n, op.gid, op.wires.data(), static_cast<int>(op.wires.size()), cw, cv,
-                               static_cast<int>(op.ctrls.size()), !op.inverse, op.params.data());
-            } else {
-                LQ_ABORT_IF(!op.ctrls.empty(), (std::string("Controlled operation does not exist for ") + op.name).c_str());
-                std::vector<cd> mm = op.matrix;
-                const size_t d = static_cast<size_t>(1) << op.wires.size();
-                LQ_ABORT_IF(mm.size() != d * d, (std::string("Operation does not exist for ") + op.name).c_str());
-                if (!op.inverse) {
-                    std::vector<cd> t(d * d);
-                    for (size_t r = 0; r < d; ++r)
-                        for (size_t c = 0; c < d; ++c) t[c * d + r] = std::conj(mm[r * d + c]);
-                    mm.swap(t);
-                }
-                std::vector<int> tb;
-                for (auto w : op.wires) tb.push_back(phys_bit(n, w));
-                u = lower_matrix(std::move(mm), tb, 0, 0, false);
-            }
-            u.smask = all;
-            prog.push_back(std::move(u));
-        }
-        // ops after the last dot change no result
-        prog.resize(std::max(last_dot, static_cast<size_t>(exp_slots)));
+        const std::vector<double> scales = backward_sweep(sv.n(), ops, tp, m, exp_slots, prog);
         const int nslots = exp_slots + static_cast<int>(ntp * m);
         const auto v = sv.run(prog, states, nslots);
         for (size_t j = 0; j < m; ++j) {
@@ -1020,6 +1026,27 @@ void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std
         }
     }
     std::copy(out.begin(), out.end(), jac);
+}
+
+void vector_jacobian_product(StateVector &sv, const OpsList &ops, const void *dy_host, uint64_t len,
+                             const std::vector<uint64_t> &tp_in, bool apply_operations, cd *out) {
+    LQ_ABORT_IF(len != sv.length(), "Assertion failed: dy.size() == jd.getSizeStateVec()");
+    std::vector<uint64_t> tp = tp_in;
+    std::sort(tp.begin(), tp.end());
+    const size_t ntp = tp.size();
+    for (size_t t = 0; t < ntp; ++t) out[t] = cd(0.0, 0.0);
+    if (ntp == 0) return;
+    sv.flush();
+    void *lam = sv.workspace(0);
+    void *mu = sv.workspace(2);
+    hip_check(hipMemcpyAsync(lam, sv.data(), sv.bytes(), hipMemcpyDeviceToDevice, sv.stream()), "memcpy");
+    hip_check(hipMemcpyAsync(mu, dy_host, sv.bytes(), hipMemcpyHostToDevice, sv.stream()), "memcpy(dy)");
+    if (apply_operations) apply_forward(sv, ops, lam);
+    std::vector<LOp> prog;
+    const std::vector<double> scales = backward_sweep(sv.n(), ops, tp, 1, 0, prog);
+    const auto v = sv.run(prog, {lam, mu}, static_cast<int>(ntp));
+    // w_t = i s <G mu | lambda> = i s <mu | G lambda>
+    for (size_t t = 0; t < ntp; ++t) out[t] = cd(0.0, scales[t]) * v[t];
 }
 
 } // namespace lqamd

@@ -185,5 +185,10 @@ class StateVector {
 
 void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std::shared_ptr<Obs>> &obs,
                       const std::vector<uint64_t> &tp, bool apply_operations, double *jac, double *expvals);
+// w_t = <dy | d psi / d theta_t> for the trainable parameters tp
+// (lq/algorithms/VectorJacobianProduct.hpp:43-160); dy is a host array of
+// len amplitudes in the state's precision.
+void vector_jacobian_product(StateVector &sv, const OpsList &ops, const void *dy_host, uint64_t len,
+                             const std::vector<uint64_t> &tp, bool apply_operations, cd *out);
 
 } // namespace lqamd

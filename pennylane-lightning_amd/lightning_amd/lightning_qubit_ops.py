@@ -25,6 +25,7 @@ and run as fused passes when a result is read.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import types
 from typing import Sequence
 
@@ -514,9 +515,40 @@ class AdjointJacobian:
         return out
 
 
-# Precision-suffixed names, as in the reference module
-observables = types.SimpleNamespace()
-algorithms = types.SimpleNamespace()
+class VectorJacobianProduct:
+    """VectorJacobianProductC*()(sv, ops, dy, trainable) -> complex array of
+    w_t = <dy | d psi / d theta_t> (LQubitBindings.hpp:412-457,
+    VectorJacobianProduct.hpp:43-160)."""
+
+    def __call__(self, sv: _StateVectorBase, operations: OpsStruct, dy, trainableParams,
+                 apply_operations: bool = False):
+        cdt = np.complex64 if sv._bits == 64 else np.complex128
+        d = np.ascontiguousarray(np.asarray(dy).reshape(-1), dtype=cdt)
+        tp = np.ascontiguousarray(np.asarray(sorted(int(t) for t in trainableParams), dtype=np.uint64))
+        out = np.zeros(max(2 * tp.size, 2), dtype=np.float64)
+        tpp = tp.ctypes.data_as(C.POINTER(C.c_uint64)) if tp.size else None
+        check(lib.lq_vector_jacobian_product(sv._h, operations._h, d.ctypes.data_as(C.c_void_p), d.size, tpp,
+                                             tp.size, int(bool(apply_operations)),
+                                             out.ctypes.data_as(C.POINTER(C.c_double))))
+        w = out[: 2 * tp.size].view(np.complex128)
+        return w.astype(cdt)
+
+
+def allocate_aligned_array(size: int, dtype, zero_init: bool = False) -> np.ndarray:
+    """1-D host array of ``size`` elements whose data starts on a 64-byte
+    boundary (Bindings.hpp:381-456)."""
+    dt = np.dtype(dtype)
+    raw = (np.zeros if zero_init else np.empty)(size * dt.itemsize + 64, dtype=np.uint8)
+    off = (-raw.ctypes.data) % 64
+    return raw[off: off + size * dt.itemsize].view(dt)
+
+
+# Precision-suffixed names, as in the reference module; the two namespaces are
+# also importable as submodules (``from ...lightning_qubit_ops.algorithms import``)
+observables = types.ModuleType(__name__ + ".observables")
+algorithms = types.ModuleType(__name__ + ".algorithms")
+sys.modules[observables.__name__] = observables
+sys.modules[algorithms.__name__] = algorithms
 for _bits in (64, 128):
     for _cls in (NamedObs, HermitianObs, TensorProdObs, Hamiltonian):
         _sub = type(f"{_cls.__name__}C{_bits}", (_cls,), {})
@@ -524,6 +556,9 @@ for _bits in (64, 128):
         globals()[_sub.__name__] = _sub
     _ops = type(f"OpsStructC{_bits}", (OpsStruct,), {})
     _adj = type(f"AdjointJacobianC{_bits}", (AdjointJacobian,), {})
+    _vjp = type(f"VectorJacobianProductC{_bits}", (VectorJacobianProduct,), {})
+    setattr(algorithms, _vjp.__name__, _vjp)
+    globals()[_vjp.__name__] = _vjp
     setattr(algorithms, _ops.__name__, _ops)
     setattr(algorithms, _adj.__name__, _adj)
     setattr(algorithms, f"create_ops_listC{_bits}",

@@ -5,7 +5,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG_DIR = os.path.join(ROOT, "pennylane-lightning_amd")
-for p in (ROOT, PKG_DIR):
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+for p in (ROOT, PKG_DIR, GOLDEN_DIR):
     if p not in sys.path:
         sys.path.insert(0, p)
 

@@ -267,3 +267,33 @@ def test_adjoint_apply_operations_and_edge_cases():
     bad = [W.CircuitOp("Rot", [0], [0.1, 0.2, 0.3])]
     with pytest.raises(RuntimeError, match="adjoint differentiation"):
         adj(sv, [Z], W.to_ops(bad), [0])
+
+
+@pytest.mark.parametrize("bits", [128, 64])
+@pytest.mark.parametrize("n", [6, 13])
+def test_vector_jacobian_product(bits, n):
+    rng = np.random.default_rng(60 + n)
+    ops = W.strongly_entangling(n, 2, seed=n)
+    ops += [W.CircuitOp("CRZ", [0, 3], [0.4]), W.CircuitOp("IsingXX", [1, 2], [0.9], inverse=True),
+            W.CircuitOp("RX", [4], [0.2], ctrls=[5], cvals=[False]),
+            W.CircuitOp("QubitUnitary", [2, 4], matrix=O.gate_matrix("CNOT")),
+            W.CircuitOp("PhaseShift", [3], [1.3])]
+    npar = sum(1 for op in ops if op.params)
+    tp = sorted(int(t) for t in rng.choice(npar, size=npar - 4, replace=False))
+    dy = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    dy /= np.linalg.norm(dy)
+    sv = SV[bits](n)
+    W.apply_circuit(sv, ops)
+    vjp = getattr(L.algorithms, f"VectorJacobianProductC{bits}")()
+    w = vjp(sv, W.to_ops(ops, bits), dy.astype(CDT[bits]), tp)
+    assert w.dtype == CDT[bits] and w.shape == (len(tp),)
+    st = oracle_run(n, ops)
+    exp = O.vector_jacobian_product(st, n, to_oracle_ops(ops), dy.astype(CDT[bits]).astype(np.complex128), tp)
+    np.testing.assert_allclose(w, exp, atol=1e-10 if bits == 128 else 5e-4)
+    # apply_operations=True starts from the initial state
+    sv0 = SV[bits](n)
+    w0 = vjp(sv0, W.to_ops(ops, bits), dy.astype(CDT[bits]), tp, apply_operations=True)
+    np.testing.assert_allclose(w0, exp, atol=1e-10 if bits == 128 else 5e-4)
+    np.testing.assert_allclose(sv.state(), st, atol=TOL[bits] * 10)
+    with pytest.raises(L.LightningException):
+        vjp(sv, W.to_ops(ops, bits), dy[:-1].astype(CDT[bits]), tp)

@@ -392,3 +392,61 @@ def test_adjoint_vs_reference():
     # single-observable path (AdjointJacobianLQubit.hpp:225-319)
     np.testing.assert_allclose(O.adjoint_jacobian(st, n, ops, [H], tp), r.adjoint_jacobian(ops, [H], tp),
                                atol=1e-12)
+
+
+def _vjp_circuit():
+    rng = np.random.default_rng(41)
+    n = 5
+    ops = O.circuit_strongly_entangling(n, 2, seed=7)
+    ops.append(O.Op("CRZ", [0, 3], [0.4]))
+    ops.append(O.Op("IsingXX", [1, 2], [0.9], inverse=True))
+    ops.append(O.Op("RX", [4], [0.2], ctrls=[1], cvals=[0]))
+    ops.append(O.Op("QubitUnitary", [2, 4], [], matrix=O.gate_matrix("CNOT")))
+    ops.append(O.Op("PhaseShift", [3], [1.3]))
+    npar = sum(1 for op in ops if op.params)
+    tp = sorted(rng.choice(npar, size=npar - 3, replace=False).tolist())
+    dy = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    return n, ops, tp, dy
+
+
+def test_vjp_vs_finite_difference():
+    """w_t = <dy | d psi / d theta_t> by central differences of the state."""
+    n, ops, tp, dy = _vjp_circuit()
+    st = O.apply_ops(O.zero_state(n), n, ops)
+    w = O.vector_jacobian_product(st, n, ops, dy, tp)
+    par = [op.params[0] for op in ops if op.params]
+
+    def state(ps):
+        it = iter(ps)
+        ops2 = [O.Op(op.name, op.wires, [next(it)] if op.params else [], op.inverse,
+                     op.matrix, op.ctrls, op.cvals) for op in ops]
+        return O.apply_ops(O.zero_state(n), n, ops2)
+
+    h = 1e-6
+    for k, t in enumerate(tp):
+        p1, p2 = list(par), list(par)
+        p1[t] += h
+        p2[t] -= h
+        d = (state(p1) - state(p2)) / (2 * h)
+        assert abs(np.vdot(dy, d) - w[k]) < 1e-8
+
+
+def test_vjp_matches_adjoint_for_observable_cotangent():
+    """With dy = O psi (O Hermitian), d<O>/d theta_t = 2 Re w_t."""
+    n, ops, tp, _ = _vjp_circuit()
+    st = O.apply_ops(O.zero_state(n), n, ops)
+    H = O.random_pauli_hamiltonian(n, 10, seed=8, probs_ixyz=(0.4, 0.2, 0.2, 0.2))
+    dy = O.obs_apply(H, st.copy(), n)
+    w = O.vector_jacobian_product(st, n, ops, dy, tp)
+    jac = O.adjoint_jacobian(st, n, ops, [H], tp)[0]
+    np.testing.assert_allclose(2 * w.real, jac, atol=1e-12)
+
+
+@needs_ref
+def test_vjp_vs_reference():
+    n, ops, tp, dy = _vjp_circuit()
+    st = O.apply_ops(O.zero_state(n), n, ops)
+    r = ref.RefStateVector(n)
+    r.apply_ops(ops)
+    np.testing.assert_allclose(O.vector_jacobian_product(st, n, ops, dy, tp),
+                               r.vector_jacobian_product(ops, dy, tp), atol=1e-12)
