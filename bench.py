@@ -186,6 +186,182 @@ def run_adjoint(args):
     print(json.dumps(out))
 
 
+def run_serving(args):
+    """Serving workload: requests with n in [10, 28] (P(n) ~ 2^-((n-10)/2)),
+    depth 4..16 hardware-efficient layers in adjoint form, 50-term Pauli
+    Hamiltonian; each asks for the energy and its full adjoint gradient
+    (complex128). Poisson arrivals at each rate of a sweep, through the
+    batched server and through the plain one-request-at-a-time path."""
+    from lightning_amd import serving as S
+
+    reqs = S.serving_requests(args.requests, seed=0)
+    t0 = time.perf_counter()
+    for r in reqs:
+        r.handles()
+    setup_s = time.perf_counter() - t0
+    warm = [r for r in reqs if r.n <= 14][:64]
+    rows = []
+    qps_plain = [float(x) for x in (args.qps.split(",") if args.qps else ["100", "200", "400"])]
+    qps_batched = [float(x) for x in (args.qps.split(",") if args.qps else ["500", "1000", "2000", "4000"])]
+    for batched, sweep in ((False, qps_plain), (True, qps_batched)):
+        srv = S.Server(bits=128, batched=batched)
+        srv.replay(warm, np.zeros(len(warm)))
+        for q in sweep:
+            res = srv.replay(reqs, S.poisson_arrivals(len(reqs), q, seed=7))
+            summ = S.summarize(res, len(reqs))
+            summ.update({"mode": "batched" if batched else "plain", "offered_qps": q})
+            rows.append(summ)
+            print(json.dumps({"serving_point": summ}), file=sys.stderr, flush=True)
+        srv.close()
+    plain = [r for r in rows if r["mode"] == "plain"]
+    best_plain = max(plain, key=lambda r: r["req_per_s"])
+    ok = [r for r in rows if r["mode"] == "batched" and r["p99_ms"] <= best_plain["p99_ms"]]
+    best_b = max(ok, key=lambda r: r["req_per_s"]) if ok else None
+    ratio = best_b["req_per_s"] / best_plain["req_per_s"] if best_b else None
+    out = {
+        "metric": "adjoint-grad req/s at equal p99, batched vs plain GPU path",
+        "value": round(ratio, 3) if ratio else None, "unit": "x plain throughput at p99 <= plain p99",
+        "n_gpus": 1, "steps": len(rows), "warmup": len(warm), "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+        "config": {"workload": f"{args.requests} requests, n 10..28 with P(n) ~ 2^-((n-10)/2), depth 4..16 HEA "
+                               f"(RZ RY RZ + CZ), 50-term Pauli Hamiltonian, energy + full adjoint gradient",
+                   "requests": args.requests, "jit": os.environ.get("LQ_JIT", "1")},
+        "plain_best": best_plain, "batched_at_equal_p99": best_b, "sweep": rows,
+        "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(out))
+
+
+def run_qft33(args):
+    """33-qubit QFT (33 H, 528 ControlledPhaseShift(pi/2^k), 16 SWAP), 8
+    brickwork layers of Haar-random 4x4 unitaries (applyMatrix) each followed
+    by one 3-controlled X on random wires; outputs <Z_i> for all wires and the
+    marginal probabilities of wires 0-9. complex128 (128 GiB) timed, then the
+    complex64 run (64 GiB) and its fidelity and output errors against it."""
+    import torch
+    from lightning_amd import lightning_qubit_ops as L
+    from lightning_amd import workloads as W
+
+    n = args.qubits if args.qubits != 30 else 33
+    ops = W.qft_brickwork_mcx(n, layers=8, mcx_per_layer=1, seed=5)
+
+    def outputs(sv, M):
+        z = M.expval_named_batch(["PauliZ"] * n, list(range(n)))
+        return np.asarray(z), np.asarray(M.probs(list(range(10))))
+
+    sv = L.StateVectorC128(n)
+    M = L.MeasurementsC128(sv)
+
+    def step():
+        sv.resetStateVector()
+        W.apply_circuit(sv, ops)
+        return outputs(sv, M)
+
+    for _ in range(args.warmup):
+        step()
+    L.jit_wait()
+    sv.synchronize()
+    sv.pass_counts(reset=True)
+    sv.profile(True)
+    sv.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        z128, p128 = step()
+    sv.synchronize()
+    t1 = time.perf_counter()
+    st = sv.profile_read(reset=True)
+    sv.profile(False)
+    pc = sv.pass_counts(reset=True)
+    ms = (t1 - t0) * 1e3 / args.steps
+    sv64 = L.StateVectorC64(n)
+    W.apply_circuit(sv64, ops)
+    z64, p64 = outputs(sv64, L.MeasurementsC64(sv64))
+    ov = sv.overlap(sv64)
+    n64 = sv64.overlap(sv64).real
+    fid = abs(ov) ** 2 / n64
+    out = {
+        "metric": f"{n}-qubit QFT + Haar brickwork + MCX", "value": round(st["bytes"] / (st["ms"] * 1e-3) / 1e9, 1)
+        if st["ms"] else None, "unit": "GB/s (pass bytes / pass time)", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+        "config": {"workload": f"QFT({n}) + 8 Haar 4x4 brickwork layers + 8 3-controlled X, <Z_i> + probs(0..9)",
+                   "n_qubits": n, "ops": len(ops)},
+        "passes_per_step": st["passes"] / args.steps, "hbm_frac": round(st["bytes"] / (st["ms"] * 1e-3) / HBM_PEAK, 4)
+        if st["ms"] else None, "specialised_passes": pc["specialised"], "generic_passes": pc["generic"],
+        "complex64": {"fidelity": fid, "max_abs_dZ": float(np.abs(z64 - z128).max()),
+                      "max_abs_dprobs": float(np.abs(p64 - p128).max()), "norm64": n64},
+        "z_mean_abs": float(np.abs(z128).mean()),
+    }
+    print(json.dumps(out))
+
+
+def run_precision(args):
+    """complex64 against complex128 for each workload: state fidelity and
+    output errors (expvals, probabilities, energies, gradients)."""
+    from lightning_amd import lightning_qubit_ops as L
+    from lightning_amd import workloads as W
+    from lightning_amd import serving as S
+
+    rows = {}
+
+    def pair(n, ops):
+        a = L.StateVectorC128(n)
+        W.apply_circuit(a, ops)
+        b = L.StateVectorC64(n)
+        W.apply_circuit(b, ops)
+        return a, b
+
+    # 20-qubit RX/RY/RZ + CNOT, 10 layers: <Z_i> and probs
+    n = 20
+    a, b = pair(n, W.layers_rxyz_cnot(n, 10, seed=0))
+    ma, mb = L.MeasurementsC128(a), L.MeasurementsC64(b)
+    za = np.array([ma.expval("PauliZ", [w]) for w in range(n)])
+    zb = np.array([mb.expval("PauliZ", [w]) for w in range(n)])
+    rows["rxyz_cnot_20"] = {"fidelity": abs(a.overlap(b)) ** 2, "max_abs_dZ": float(np.abs(za - zb).max()),
+                            "max_abs_dprobs": float(np.abs(ma.probs() - mb.probs()).max())}
+    del a, b, ma, mb
+    # 30-qubit HEA depth 20: <Z0 Z1>, <X29>
+    n = 30
+    a, b = pair(n, W.hea_rot_cz(n, 20, seed=1))
+    zz = [L.TensorProdObsC128([L.NamedObsC128("PauliZ", [0]), L.NamedObsC128("PauliZ", [1])]),
+          L.TensorProdObsC64([L.NamedObsC64("PauliZ", [0]), L.NamedObsC64("PauliZ", [1])])]
+    ea = [L.MeasurementsC128(a).expval(zz[0]), L.MeasurementsC128(a).expval("PauliX", [29])]
+    eb = [L.MeasurementsC64(b).expval(zz[1]), L.MeasurementsC64(b).expval("PauliX", [29])]
+    rows["hea_30"] = {"fidelity": abs(a.overlap(b)) ** 2, "abs_dZ0Z1": abs(ea[0] - eb[0]), "abs_dX29": abs(ea[1] - eb[1])}
+    del a, b
+    # 24-qubit adjoint: energy and the 432 gradients
+    n = 24
+    ops = W.strongly_entangling(n, 6, seed=2)
+    coeffs, words = W.random_pauli_hamiltonian(n, 200, seed=3)
+    tp = list(range(sum(1 for o in ops if o.params)))
+    res = []
+    for bits in (128, 64):
+        sv = (L.StateVectorC128 if bits == 128 else L.StateVectorC64)(n)
+        W.apply_circuit(sv, ops)
+        adj = (L.algorithms.AdjointJacobianC128 if bits == 128 else L.algorithms.AdjointJacobianC64)()
+        j, e = adj(sv, [W.hamiltonian_obs(coeffs, words, bits)], W.to_ops(ops, bits), tp, return_expvals=True)
+        res.append((np.asarray(j, dtype=np.float64), float(e[0])))
+        del sv
+    rows["adjoint_24"] = {"abs_denergy": abs(res[0][1] - res[1][1]),
+                          "max_abs_dgrad": float(np.abs(res[0][0] - res[1][0]).max()),
+                          "rel_dgrad_norm": float(np.linalg.norm(res[0][0] - res[1][0]) / np.linalg.norm(res[0][0]))}
+    # serving sample (n <= 16, packed kernel): energies and gradients
+    reqs = S.serving_requests(64, seed=11, n_max=16)
+    outs = []
+    for bits in (128, 64):
+        for r in reqs:
+            r.bits, r._ops_h, r._ham_h = bits, None, None
+        srv = S.Server(bits=bits, batched=True)
+        outs.append(srv.replay(reqs, np.zeros(len(reqs))))
+        srv.close()
+    dg = max(float(np.abs(x - y).max()) for x, y in zip(outs[0]["grads"], outs[1]["grads"]))
+    rows["serving_sample_64"] = {"max_abs_denergy": float(np.abs(outs[0]["energies"] - outs[1]["energies"]).max()),
+                                 "max_abs_dgrad": dg}
+    print(json.dumps({"metric": "complex64 error against complex128", "rows": rows, "dtype": "complex64",
+                      "n_gpus": 1, "data": "synthetic"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,8 +374,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain streaming kernel (4 GiB read + 4 GiB written per launch)")
-    ap.add_argument("--workload", choices=["hea30", "adjoint24"], default="hea30",
-                    help="hea30: the gate-application line (default); adjoint24: the 24-qubit adjoint gradient")
+    ap.add_argument("--workload", choices=["hea30", "adjoint24", "serving", "qft33", "precision"], default="hea30",
+                    help="hea30: the gate-application line (default); adjoint24: the 24-qubit adjoint gradient; "
+                         "serving: energy + gradient requests, batched vs plain; qft33: 33-qubit QFT + Haar "
+                         "brickwork + 3-controlled X with complex64 fidelity; precision: complex64 error per "
+                         "workload against the complex128 run")
+    ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -217,6 +398,12 @@ def main():
         return
     if args.workload == "adjoint24":
         return run_adjoint(args)
+    if args.workload == "serving":
+        return run_serving(args)
+    if args.workload == "qft33":
+        return run_qft33(args)
+    if args.workload == "precision":
+        return run_precision(args)
     from lightning_amd import lightning_qubit_ops as L
     from lightning_amd import workloads as W
 

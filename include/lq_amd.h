@@ -148,8 +148,50 @@ int lq_sv_profile(lq_sv_t sv, int enable);
 int lq_sv_profile_read(lq_sv_t sv, int reset, uint64_t *passes, double *ms, double *bytes, double *max_ms);
 int lq_sv_stream(lq_sv_t sv, void **out);
 
+/* <a|b> of two states of the same size, either precision, accumulated in FP64
+ * (no reference counterpart; used for the complex64 fidelity reports) */
+int lq_sv_overlap(lq_sv_t a, lq_sv_t b, double *re, double *im);
+
 /* free the adjoint's work states held by the handle */
 int lq_sv_release_workspace(lq_sv_t sv);
+
+/* ---- Request server (no reference counterpart) ------------------------------
+ * Energy + adjoint-gradient requests (ops list, Pauli-sum Hamiltonian,
+ * trainable parameters) served with continuous batching on one GPU: requests
+ * of up to 16 qubits run many per launch, one workgroup each, with lambda and
+ * H lambda held in LDS (complex128: n <= 12, complex64: n <= 13) or in a
+ * global scratch area; larger ones run through the engine on concurrent workers,
+ * admitted while 3 x 2^n amplitudes fit the HBM budget. batched = 0 gives the
+ * plain path: one request at a time, in arrival order, through the engine. */
+typedef struct lq_server_s *lq_server_t;
+typedef struct {
+    int dtype;               /* LQ_C64 / LQ_C128 */
+    int batched;             /* 1: continuous batching; 0: plain FIFO */
+    uint64_t hbm_budget;     /* bytes; 0: 250 GiB */
+    int small_max_qubits;    /* -1: the packed kernel's limit */
+    int max_batch;           /* requests per packed launch (0: 4096) */
+    int batch_streams;       /* packed launches in flight (0: 2) */
+    int large_workers;       /* concurrent engine requests (0: 4) */
+    int prep_threads;        /* host threads building packed programs (0: 4) */
+} lq_server_config;
+int lq_server_create(const lq_server_config *cfg, lq_server_t *out);
+int lq_server_destroy(lq_server_t srv);
+int lq_server_submit(lq_server_t srv, uint32_t num_qubits, lq_ops_t ops, lq_obs_t ham, const uint64_t *trainable,
+                     uint32_t ntp, uint64_t *id);
+/* Wait for request `id` (timeout in seconds); *done = 0 on timeout. grad
+ * receives ntp values; latency_ms is submit -> result; path 0 = packed,
+ * 1 = engine. A failed request returns its error. */
+int lq_server_wait(lq_server_t srv, uint64_t id, double timeout_s, int *done, double *energy, double *grad,
+                   uint32_t ntp, double *latency_ms, int *path);
+/* Submit `count` requests at arrival_s[i] seconds after the call (request i:
+ * n[i] qubits, ops[i], ham[i], trainable tp[tp_off[i] .. tp_off[i+1])) and
+ * collect every result: energies[i], grads[tp_off[i] ..], latency_ms[i],
+ * path[i]; *wall_s = first arrival -> last completion. */
+int lq_server_replay(lq_server_t srv, uint32_t count, const uint32_t *n, const lq_ops_t *ops, const lq_obs_t *ham,
+                     const uint64_t *tp, const uint64_t *tp_off, const double *arrival_s, double *energies,
+                     double *grads, double *latency_ms, int *path, double *wall_s);
+/* submitted, completed, packed, engine, packed launches, HBM bytes held */
+int lq_server_stats(lq_server_t srv, uint64_t *out6);
 
 /* ---- Kernel specialisation (no reference counterpart) ----------------------
  * Register tile passes of states with at least LQ_JIT_MIN_QUBITS qubits

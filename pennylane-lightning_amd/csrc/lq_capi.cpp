@@ -1,13 +1,18 @@
 // extern "C" boundary (include/lq_amd.h) over the engine.
 #include "lq_amd.h"
 
+#include <atomic>
+#include <chrono>
 #include <cstring>
+#include <mutex>
+#include <thread>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "lq_engine.hpp"
 #include "lq_jit.hpp"
+#include "lq_serve.hpp"
 
 using namespace lqamd;
 
@@ -329,8 +334,142 @@ int lq_sv_stream(lq_sv_t sv, void **out) {
     return guard([&] { *out = static_cast<void *>(S(sv).stream()); });
 }
 
+int lq_sv_overlap(lq_sv_t a, lq_sv_t b, double *re, double *im) {
+    return guard([&] {
+        const cd v = S(a).overlap(S(b));
+        *re = v.real();
+        *im = v.imag();
+    });
+}
+
 int lq_sv_release_workspace(lq_sv_t sv) {
     return guard([&] { S(sv).release_workspace(); });
+}
+
+struct lq_server_s {
+    std::unique_ptr<lqamd::Server> srv;
+};
+
+int lq_server_create(const lq_server_config *cfg, lq_server_t *out) {
+    return guard([&] {
+        LQ_ABORT_IF(!cfg || !out, "null argument");
+        LQ_ABORT_IF(cfg->dtype != LQ_C64 && cfg->dtype != LQ_C128, "invalid dtype");
+        lqamd::ServerConfig c;
+        c.dtype = cfg->dtype;
+        c.batched = cfg->batched;
+        if (cfg->hbm_budget) c.hbm_budget = cfg->hbm_budget;
+        c.small_max_qubits = cfg->small_max_qubits;
+        if (cfg->max_batch > 0) c.max_batch = cfg->max_batch;
+        if (cfg->batch_streams > 0) c.batch_streams = cfg->batch_streams;
+        if (cfg->large_workers > 0) c.large_workers = cfg->large_workers;
+        if (cfg->prep_threads > 0) c.prep_threads = cfg->prep_threads;
+        auto *h = new lq_server_s;
+        h->srv = std::make_unique<lqamd::Server>(c);
+        *out = h;
+    });
+}
+
+int lq_server_destroy(lq_server_t srv) {
+    return guard([&] { delete srv; });
+}
+
+int lq_server_submit(lq_server_t srv, uint32_t num_qubits, lq_ops_t ops, lq_obs_t ham, const uint64_t *trainable,
+                     uint32_t ntp, uint64_t *id) {
+    return guard([&] {
+        LQ_ABORT_IF(!srv || !ops || !ham || !id, "null argument");
+        std::vector<uint64_t> tp(trainable, trainable + ntp);
+        *id = srv->srv->submit(static_cast<int>(num_qubits), ops->list, ham->obs, tp);
+    });
+}
+
+int lq_server_wait(lq_server_t srv, uint64_t id, double timeout_s, int *done, double *energy, double *grad,
+                   uint32_t ntp, double *latency_ms, int *path) {
+    return guard([&] {
+        LQ_ABORT_IF(!srv || !done, "null argument");
+        lqamd::ServeResult r;
+        *done = srv->srv->wait(id, timeout_s, &r) ? 1 : 0;
+        if (!*done) return;
+        LQ_ABORT_IF(!r.error.empty(), r.error.c_str());
+        if (energy) *energy = r.energy;
+        if (grad) {
+            LQ_ABORT_IF(r.grad.size() != ntp, "gradient size mismatch");
+            std::copy(r.grad.begin(), r.grad.end(), grad);
+        }
+        if (latency_ms) *latency_ms = r.latency_ms;
+        if (path) *path = r.path;
+    });
+}
+
+int lq_server_replay(lq_server_t srv, uint32_t count, const uint32_t *n, const lq_ops_t *ops, const lq_obs_t *ham,
+                     const uint64_t *tp, const uint64_t *tp_off, const double *arrival_s, double *energies,
+                     double *grads, double *latency_ms, int *path, double *wall_s) {
+    return guard([&] {
+        LQ_ABORT_IF(!srv, "null server");
+        std::vector<uint64_t> ids(count, 0);
+        std::atomic<uint32_t> submitted{0};
+        std::string err;
+        std::mutex err_mu;
+        using Clock = std::chrono::steady_clock;
+        const auto start = Clock::now();
+        Clock::time_point last_done = start;
+        std::thread collector([&] {
+          try {
+            for (uint32_t i = 0; i < count; ++i) {
+                while (submitted.load(std::memory_order_acquire) <= i) {
+                    {
+                        std::lock_guard<std::mutex> lk(err_mu);
+                        if (!err.empty()) return;
+                    }
+                    std::this_thread::sleep_for(std::chrono::microseconds(50));
+                }
+                lqamd::ServeResult r;
+                while (!srv->srv->wait(ids[i], 1.0, &r)) {
+                }
+                if (!r.error.empty()) {
+                    std::lock_guard<std::mutex> lk(err_mu);
+                    if (err.empty()) err = r.error;
+                }
+                energies[i] = r.energy;
+                std::copy(r.grad.begin(), r.grad.end(), grads + tp_off[i]);
+                latency_ms[i] = r.latency_ms;
+                path[i] = r.path;
+                last_done = Clock::now();
+            }
+          } catch (const std::exception &ex) {
+            std::lock_guard<std::mutex> lk(err_mu);
+            if (err.empty()) err = ex.what();
+          }
+        });
+        try {
+            for (uint32_t i = 0; i < count; ++i) {
+                std::this_thread::sleep_until(start + std::chrono::duration_cast<Clock::duration>(
+                                                          std::chrono::duration<double>(arrival_s[i])));
+                std::vector<uint64_t> t(tp + tp_off[i], tp + tp_off[i + 1]);
+                ids[i] = srv->srv->submit(static_cast<int>(n[i]), ops[i]->list, ham[i]->obs, t);
+                submitted.store(i + 1, std::memory_order_release);
+            }
+        } catch (const std::exception &ex) {
+            std::lock_guard<std::mutex> lk(err_mu);
+            if (err.empty()) err = ex.what();
+        }
+        collector.join();
+        LQ_ABORT_IF(!err.empty(), err.c_str());
+        *wall_s = std::chrono::duration<double>(last_done - (start + std::chrono::duration_cast<Clock::duration>(
+                                                                         std::chrono::duration<double>(arrival_s[0]))))
+                      .count();
+    });
+}
+
+int lq_server_stats(lq_server_t srv, uint64_t *out6) {
+    return guard([&] {
+        const auto s = srv->srv->stats();
+        out6[0] = s.submitted;
+        out6[1] = s.completed;
+        out6[2] = s.packed;
+        out6[3] = s.engine;
+        out6[4] = s.launches;
+        out6[5] = s.hbm_in_use;
+    });
 }
 
 int lq_jit_wait(void) {

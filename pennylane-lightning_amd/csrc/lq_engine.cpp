@@ -435,8 +435,7 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     if (jc.enabled && n_ >= jc.min_qubits) {
         for (size_t p = 0; p < passes.size(); ++p) {
             if (!passes[p].reg) continue;
-            const std::string src = jit_pass_source(passes[p], n_, dtype_, ns);
-            if (!src.empty()) fns[p] = jit_lookup(src);
+            fns[p] = jit_lookup_plan(passes[p], n_, dtype_, ns);
         }
     }
 
@@ -651,6 +650,28 @@ double StateVector::norm2() {
         s = t;
     }
     return s + c;
+}
+
+cd StateVector::overlap(StateVector &other) {
+    LQ_ABORT_IF(other.n_ != n_, "states of different sizes");
+    flush();
+    other.synchronize();
+    const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
+    partials_.ensure(nb * sizeof(double2));
+    launch_overlap(d_, dtype_, other.d_, other.dtype_, n_, static_cast<double2 *>(partials_.p), nb, stream_);
+    std::vector<double2> h(nb);
+    hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    double sr = 0.0, cr = 0.0, si = 0.0, ci = 0.0;
+    for (const auto &v : h) {
+        double t = sr + v.x;
+        cr += (std::fabs(sr) >= std::fabs(v.x)) ? (sr - t) + v.x : (v.x - t) + sr;
+        sr = t;
+        t = si + v.y;
+        ci += (std::fabs(si) >= std::fabs(v.y)) ? (si - t) + v.y : (v.y - t) + si;
+        si = t;
+    }
+    return cd(sr + cr, si + ci);
 }
 
 void StateVector::normalize() {
@@ -891,8 +912,6 @@ double StateVector::expval_obs(const Obs &o) {
 // ---------------------------------------------------------------------------
 // Adjoint Jacobian (lq/algorithms/AdjointJacobianLQubit.hpp:225-491)
 // ---------------------------------------------------------------------------
-namespace {
-
 // Lowered op of an OpsList entry, as applied forward (inverse flag honoured)
 // or, with `dagger`, as its adjoint U^dagger
 // (core/algorithms/AdjointJacobianBase.hpp:108-127).
@@ -980,7 +999,6 @@ std::vector<double> backward_sweep(int n, const OpsList &ops, const std::vector<
     return scales;
 }
 
-} // namespace
 
 void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std::shared_ptr<Obs>> &obs,
                       const std::vector<uint64_t> &tp_in, bool apply_operations, double *jac, double *expvals) {

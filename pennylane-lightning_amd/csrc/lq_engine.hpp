@@ -149,6 +149,10 @@ class StateVector {
         double bytes = 0.0; // algorithmic HBM bytes (amplitudes read + written)
         double max_ms = 0.0;
     };
+    // <this|other> for states of the same size in either precision (FP64,
+    // compensated sum of per-block partials)
+    cd overlap(StateVector &other);
+
     void profile(bool on);
     PassStats profile_read(bool reset);
     // passes launched as specialised kernels / generic tile kernels
@@ -182,6 +186,16 @@ class StateVector {
     std::vector<double> reduce_rows_to_host(uint32_t rows, uint32_t cols);
     void apply_obs_inplace(const Obs &o, void *state, std::vector<LOp> &prog);
 };
+
+// Lowered op of an OpsList entry: U, or U^dagger with `dagger`
+// (core/algorithms/AdjointJacobianBase.hpp:108-127).
+LOp lower_record(int n, const OpRec &op, bool dagger);
+// Append the backward sweep on states {lambda, b_1..b_m} to prog: for each
+// trainable op from the last, dots <b_j | G lambda> into slot
+// slot0 + t*m + j, then U^dagger on all states. Returns the generator scales
+// (negated for inverse ops) in sorted-trainable order.
+std::vector<double> backward_sweep(int n, const OpsList &ops, const std::vector<uint64_t> &tp, size_t m, int slot0,
+                                   std::vector<LOp> &prog);
 
 void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std::shared_ptr<Obs>> &obs,
                       const std::vector<uint64_t> &tp, bool apply_operations, double *jac, double *expvals);

@@ -99,7 +99,7 @@ struct Mapping {
 
 class Gen {
   public:
-    Gen(const PassPlan &p, int n, int dtype, int ns) : p_(p), n_(n), dtype_(dtype), ns_(ns) {
+    Gen(const PassPlan &p, int n, int dtype, int ns) : p_(p), dtype_(dtype), ns_(ns) {
         kt_ = static_cast<int>(p.tbits.size());
         rb_ = kt_ - 8;
         E_ = 1 << rb_;
@@ -174,7 +174,7 @@ class Gen {
 
   private:
     const PassPlan &p_;
-    int n_, dtype_, ns_, kt_, rb_, E_;
+    int dtype_, ns_, kt_, rb_, E_;
     uint32_t tsize_;
     TileGeom geom_;
     Mapping cur_{};
@@ -509,53 +509,29 @@ class Service {
         return *s;
     }
 
-    hipFunction_t lookup(const std::string &src) {
-        std::shared_ptr<JEntry> e;
-        bool compile_now = false;
+    hipFunction_t lookup(const std::string &src) { return resolve(enqueue(src)); }
+
+    // A structural key of the pass stands for its source, so a pass seen
+    // before needs no source generation.
+    hipFunction_t lookup_key(const std::string &key, const PassPlan &p, int n, int dtype, int ns) {
+        std::shared_ptr<JEntry> hit;
+        bool known = false;
         {
-            std::unique_lock<std::mutex> lk(mu_);
-            auto it = map_.find(src);
-            if (it == map_.end()) {
-                if (stopping_) return nullptr;
-                e = std::make_shared<JEntry>();
-                map_.emplace(src, e);
-                if (jit_config().sync) {
-                    compile_now = true;
-                    ++running_;
-                } else {
-                    queue_.push_back(src);
-                    ++pending_;
-                    start_workers();
-                    cv_.notify_one();
-                    return nullptr;
-                }
-            } else {
-                e = it->second;
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = keys_.find(key);
+            if (it != keys_.end()) {
+                known = true;
+                hit = it->second;
             }
         }
-        if (compile_now) {
-            compile(src, *e);
+        if (known) return hit ? resolve(hit) : nullptr;
+        const std::string src = jit_pass_source(p, n, dtype, ns);
+        std::shared_ptr<JEntry> e = src.empty() ? nullptr : enqueue(src);
+        {
             std::lock_guard<std::mutex> lk(mu_);
-            --running_;
-            done_cv_.notify_all();
+            keys_.emplace(key, e);
         }
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-        std::lock_guard<std::mutex> lk(mu_);
-        if (e->state != JState::Ready) return nullptr;
-        auto it = e->loaded.find(dev);
-        if (it != e->loaded.end()) return it->second.second;
-        hipModule_t mod = nullptr;
-        hipFunction_t fn = nullptr;
-        if (hipModuleLoadData(&mod, e->code.data()) != hipSuccess ||
-            hipModuleGetFunction(&fn, mod, "lqpass") != hipSuccess) {
-            e->state = JState::Failed;
-            ++failed_;
-            return nullptr;
-        }
-        e->loaded.emplace(dev, std::make_pair(mod, fn));
-        ++ready_;
-        return fn;
+        return e ? resolve(e) : nullptr;
     }
 
     void wait() {
@@ -585,6 +561,57 @@ class Service {
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
     std::unordered_map<std::string, std::shared_ptr<JEntry>> map_;
+    std::unordered_map<std::string, std::shared_ptr<JEntry>> keys_; // null: not specialisable
+
+    // Entry for `src`, queueing its compile (or compiling now in sync mode)
+    // the first time; null after stop().
+    std::shared_ptr<JEntry> enqueue(const std::string &src) {
+        std::shared_ptr<JEntry> e;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            auto it = map_.find(src);
+            if (it != map_.end()) return it->second;
+            if (stopping_) return nullptr;
+            e = std::make_shared<JEntry>();
+            map_.emplace(src, e);
+            if (!jit_config().sync) {
+                queue_.push_back(src);
+                ++pending_;
+                start_workers();
+                cv_.notify_one();
+                return e;
+            }
+            ++running_;
+        }
+        compile(src, *e);
+        std::lock_guard<std::mutex> lk(mu_);
+        --running_;
+        done_cv_.notify_all();
+        return e;
+    }
+
+    // The kernel of a compiled entry, loaded on the current device; null
+    // while the compile is pending or after it failed.
+    hipFunction_t resolve(const std::shared_ptr<JEntry> &e) {
+        if (!e) return nullptr;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+        std::lock_guard<std::mutex> lk(mu_);
+        if (e->state != JState::Ready) return nullptr;
+        auto it = e->loaded.find(dev);
+        if (it != e->loaded.end()) return it->second.second;
+        hipModule_t mod = nullptr;
+        hipFunction_t fn = nullptr;
+        if (hipModuleLoadData(&mod, e->code.data()) != hipSuccess ||
+            hipModuleGetFunction(&fn, mod, "lqpass") != hipSuccess) {
+            e->state = JState::Failed;
+            ++failed_;
+            return nullptr;
+        }
+        e->loaded.emplace(dev, std::make_pair(mod, fn));
+        ++ready_;
+        return fn;
+    }
     std::deque<std::string> queue_;
     int pending_ = 0, running_ = 0, ready_ = 0, failed_ = 0;
     bool started_ = false, stopping_ = false;
@@ -660,6 +687,20 @@ class Service {
 } // namespace
 
 hipFunction_t jit_lookup(const std::string &src) { return Service::get().lookup(src); }
+
+hipFunction_t jit_lookup_plan(const PassPlan &p, int n, int dtype, int ns) {
+    // every field the generated source depends on (ops and stages are
+    // zero-initialised PODs, so their bytes are a faithful key)
+    std::string key;
+    const int32_t hdr[9] = {n, dtype, ns, p.reg ? 1 : 0, static_cast<int32_t>(p.tbits.size()),
+                            static_cast<int32_t>(p.load_mask), static_cast<int32_t>(p.store_mask),
+                            static_cast<int32_t>(p.end_transpose), static_cast<int32_t>(p.slot_ids.size())};
+    key.append(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    key.append(reinterpret_cast<const char *>(p.tbits.data()), p.tbits.size() * sizeof(int));
+    key.append(reinterpret_cast<const char *>(p.ops.data()), p.ops.size() * sizeof(TileOp));
+    key.append(reinterpret_cast<const char *>(p.stages.data()), p.stages.size() * sizeof(TileStage));
+    return Service::get().lookup_key(key, p, n, dtype, ns);
+}
 void jit_wait() { Service::get().wait(); }
 void jit_shutdown() { Service::get().stop(); }
 void jit_stats(int *ready, int *failed, int *pending) { Service::get().stats(ready, failed, pending); }

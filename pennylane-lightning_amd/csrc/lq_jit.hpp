@@ -30,6 +30,9 @@ JitConfig &jit_config();
 // Kernel for `src` if compiled and loaded on the current device; otherwise
 // nullptr, after queueing the compile (or compiling now in sync mode).
 hipFunction_t jit_lookup(const std::string &src);
+// Same for a planned pass, found by its structure without generating the
+// source again once the pass has been seen.
+hipFunction_t jit_lookup_plan(const PassPlan &p, int n, int dtype, int ns);
 // Block until every queued compile has finished.
 void jit_wait();
 // Drop queued compiles and wait for those in flight; no new compiles start

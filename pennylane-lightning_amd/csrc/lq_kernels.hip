@@ -180,6 +180,22 @@ __global__ __launch_bounds__(kBlock) void norm2_kernel(const typename VecT<T>::t
     block_reduce_store(s, 0.0, red, &partials[blockIdx.x]);
 }
 
+// <a|b> of two states of either precision, accumulated in FP64
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void overlap_kernel(const typename VecT<TA>::type *__restrict__ a,
+                                                         const typename VecT<TB>::type *__restrict__ b, uint64_t N,
+                                                         double2 *partials) {
+    __shared__ double2 red[kBlock / 64];
+    double re = 0.0, im = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride) {
+        const double ax = a[i].x, ay = a[i].y, bx = b[i].x, by = b[i].y;
+        re += ax * bx + ay * by;
+        im += ax * by - ay * bx;
+    }
+    block_reduce_store(re, im, red, &partials[blockIdx.x]);
+}
+
 struct BitList {
     int32_t b[64];
 };
@@ -368,6 +384,20 @@ template <typename T> void launch_norm2(const void *a, int n, double2 *partials,
 }
 template void launch_norm2<double>(const void *, int, double2 *, uint32_t, hipStream_t);
 template void launch_norm2<float>(const void *, int, double2 *, uint32_t, hipStream_t);
+
+void launch_overlap(const void *a, int dtype_a, const void *b, int dtype_b, int n, double2 *partials, uint32_t nblocks,
+                    hipStream_t s) {
+    const uint64_t N = 1ull << n;
+#define LQ_OV(TA, TB)                                                                                          \
+    hipLaunchKernelGGL((overlap_kernel<TA, TB>), dim3(nblocks), dim3(kBlock), 0, s,                          \
+                       static_cast<const typename VecT<TA>::type *>(a), static_cast<const typename VecT<TB>::type *>(b), \
+                       N, partials)
+    if (dtype_a && dtype_b) LQ_OV(double, double);
+    else if (dtype_a) LQ_OV(double, float);
+    else if (dtype_b) LQ_OV(float, double);
+    else LQ_OV(float, float);
+#undef LQ_OV
+}
 
 template <typename T>
 void launch_probs_hist(const void *a, int n, const int *bits, int nw, double *partials, uint32_t nblocks,

@@ -70,6 +70,9 @@ void launch_pauli_sum(const void *a, void *out, const PauliTerm *terms, uint32_t
 // sum_i |a_i|^2 into per-block partials
 template <typename T>
 void launch_norm2(const void *a, int n, double2 *partials, uint32_t nblocks, hipStream_t s);
+// partials[b] for b < nblocks: per-block sums of conj(a) b (FP64)
+void launch_overlap(const void *a, int dtype_a, const void *b, int dtype_b, int n, double2 *partials, uint32_t nblocks,
+                    hipStream_t s);
 
 // marginal probabilities: out[o] over the wires given as physical bits
 // (bits[0] is the output MSB). Small nw: per-block LDS histograms into

@@ -1,0 +1,264 @@
+// Request server (see lq_serve.hpp).
+#include "lq_serve.hpp"
+
+#include <algorithm>
+#include <map>
+
+namespace lqamd {
+
+namespace {
+uint64_t request_bytes(int n, int dtype) { return 3ull * (static_cast<uint64_t>(dtype ? 16 : 8) << n); }
+} // namespace
+
+Server::Server(const ServerConfig &cfg) : cfg_(cfg) {
+    hip_check(hipGetDevice(&device_), "hipGetDevice");
+    const int lim = batch_max_qubits(cfg_.dtype);
+    cfg_.small_max_qubits = cfg_.small_max_qubits < 0 ? lim : std::min(cfg_.small_max_qubits, lim);
+    cfg_.max_batch = std::max(1, cfg_.max_batch);
+    cfg_.batch_streams = std::max(1, cfg_.batch_streams);
+    cfg_.large_workers = std::max(1, cfg_.large_workers);
+    cfg_.prep_threads = std::max(1, cfg_.prep_threads);
+    if (cfg_.batched) {
+        for (int i = 0; i < cfg_.prep_threads; ++i) threads_.emplace_back([this] { prep_loop(); });
+        threads_.emplace_back([this] { batch_loop(); });
+        for (int i = 0; i < cfg_.large_workers; ++i) threads_.emplace_back([this] { large_loop(); });
+    } else {
+        threads_.emplace_back([this] { large_loop(); });
+    }
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    cv_done_.notify_all();
+    for (auto &t : threads_) t.join();
+}
+
+uint64_t Server::submit(int n, const OpsList &ops, const std::shared_ptr<Obs> &ham, const std::vector<uint64_t> &tp) {
+    LQ_ABORT_IF(n < 1 || n > 40, "invalid number of qubits");
+    LQ_ABORT_IF(!ham, "null observable");
+    auto r = std::make_shared<Req>();
+    r->n = n;
+    r->ops = ops;
+    r->ham = ham;
+    r->tp = tp;
+    std::sort(r->tp.begin(), r->tp.end());
+    r->bytes = request_bytes(n, cfg_.dtype);
+    LQ_ABORT_IF(r->bytes > cfg_.hbm_budget, "request exceeds the HBM budget");
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        LQ_ABORT_IF(stop_, "server stopped");
+        r->id = next_id_++;
+        r->t0 = Clock::now();
+        live_[r->id] = r;
+        ++stats_.submitted;
+        if (cfg_.batched) incoming_.push_back(r);
+        else large_ready_.push_back(r);
+    }
+    cv_work_.notify_all();
+    return r->id;
+}
+
+bool Server::wait(uint64_t id, double timeout_s, ServeResult *out) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = live_.find(id);
+    LQ_ABORT_IF(it == live_.end(), "unknown request id");
+    ReqP r = it->second;
+    const auto until = Clock::now() + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(timeout_s));
+    if (!cv_done_.wait_until(lk, until, [&] { return r->done; })) return false;
+    *out = r->res;
+    live_.erase(id);
+    return true;
+}
+
+Server::Stats Server::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    Stats s = stats_;
+    s.hbm_in_use = hbm_in_use_;
+    return s;
+}
+
+void Server::complete(const ReqP &r) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        r->res.latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t0).count();
+        r->done = true;
+        ++stats_.completed;
+        if (r->res.path == 0) ++stats_.packed;
+        else ++stats_.engine;
+        r->prog = BatchProgram{};
+        r->ops = OpsList{};
+    }
+    cv_done_.notify_all();
+}
+
+void Server::prep_loop() {
+    (void)hipSetDevice(device_);
+    while (true) {
+        ReqP r;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_work_.wait(lk, [&] { return stop_ || !incoming_.empty(); });
+            if (stop_) return;
+            r = incoming_.front();
+            incoming_.pop_front();
+        }
+        bool small = false;
+        if (r->n <= cfg_.small_max_qubits) {
+            std::string why;
+            try {
+                small = build_batch_program(r->n, cfg_.dtype, r->ops, *r->ham, r->tp, &r->prog, &why);
+            } catch (const std::exception &) {
+                small = false;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            (small ? small_ready_ : large_ready_).push_back(r);
+        }
+        cv_work_.notify_all();
+    }
+}
+
+void Server::batch_loop() {
+    (void)hipSetDevice(device_);
+    const int K = cfg_.batch_streams;
+    std::vector<std::unique_ptr<BatchRunner>> runners;
+    for (int i = 0; i < K; ++i) runners.emplace_back(std::make_unique<BatchRunner>());
+    std::vector<std::vector<ReqP>> flight(K);
+    auto retire = [&](int i, bool block) {
+        if (flight[i].empty()) return false;
+        if (!block && !runners[i]->done()) return false;
+        std::vector<double> e;
+        std::vector<std::vector<double>> g;
+        std::string err;
+        try {
+            runners[i]->finish(&e, &g);
+        } catch (const std::exception &ex) {
+            err = ex.what();
+        }
+        for (size_t j = 0; j < flight[i].size(); ++j) {
+            Req &r = *flight[i][j];
+            r.res.path = 0;
+            if (err.empty()) {
+                r.res.energy = e[j];
+                r.res.grad = std::move(g[j]);
+            } else {
+                r.res.error = err;
+            }
+            complete(flight[i][j]);
+        }
+        flight[i].clear();
+        return true;
+    };
+    while (true) {
+        for (int i = 0; i < K; ++i) retire(i, false);
+        int free_slot = -1;
+        for (int i = 0; i < K && free_slot < 0; ++i)
+            if (flight[i].empty()) free_slot = i;
+        std::vector<ReqP> take;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            bool busy = false;
+            for (int i = 0; i < K; ++i) busy = busy || !flight[i].empty();
+            if (stop_ && !busy) return;
+            if (free_slot >= 0 && !small_ready_.empty() && !stop_) {
+                const size_t cnt = std::min(small_ready_.size(), static_cast<size_t>(cfg_.max_batch));
+                take.assign(small_ready_.begin(), small_ready_.begin() + static_cast<long>(cnt));
+                small_ready_.erase(small_ready_.begin(), small_ready_.begin() + static_cast<long>(cnt));
+            } else {
+                // poll the launches in flight; wake early for new work
+                cv_work_.wait_for(lk, std::chrono::microseconds(busy ? 20 : 2000), [&] {
+                    return stop_ || (free_slot >= 0 && !small_ready_.empty());
+                });
+                continue;
+            }
+        }
+        std::vector<const BatchProgram *> progs;
+        progs.reserve(take.size());
+        for (const auto &r : take) progs.push_back(&r->prog);
+        try {
+            runners[free_slot]->launch(progs, cfg_.dtype);
+            flight[free_slot] = std::move(take);
+            std::lock_guard<std::mutex> lk(mu_);
+            ++stats_.launches;
+        } catch (const std::exception &ex) {
+            for (const auto &r : take) {
+                r->res.error = ex.what();
+                complete(r);
+            }
+        }
+    }
+}
+
+void Server::run_engine(StateVector &sv, Req &r) {
+    sv.reset();
+    for (const auto &op : r.ops.ops) sv.apply_oprec(op, false);
+    std::vector<double> jac(r.tp.size());
+    double ev = 0.0;
+    if (!r.tp.empty()) {
+        adjoint_jacobian(sv, r.ops, {r.ham}, r.tp, false, jac.data(), &ev);
+    } else {
+        ev = sv.expval_obs(*r.ham);
+    }
+    r.res.energy = ev;
+    r.res.grad = std::move(jac);
+    r.res.path = 1;
+}
+
+void Server::large_loop() {
+    (void)hipSetDevice(device_);
+    // states kept per qubit count (with the adjoint's work states), counted
+    // against the HBM budget while held
+    std::map<int, std::unique_ptr<StateVector>> cache;
+    uint64_t held = 0;
+    while (true) {
+        ReqP r;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            while (true) {
+                if (stop_) {
+                    hbm_in_use_ -= held;
+                    return;
+                }
+                if (!large_ready_.empty()) {
+                    const ReqP &h = large_ready_.front();
+                    const uint64_t need = cache.count(h->n) ? 0 : h->bytes;
+                    if (hbm_in_use_ + need <= cfg_.hbm_budget) break;
+                    if (!cache.empty()) {
+                        // make room: drop this worker's other states
+                        lk.unlock();
+                        for (auto &kv : cache) kv.second.reset();
+                        cache.clear();
+                        lk.lock();
+                        hbm_in_use_ -= held;
+                        held = 0;
+                        continue;
+                    }
+                }
+                cv_work_.wait(lk);
+            }
+            r = large_ready_.front();
+            large_ready_.pop_front();
+            if (!cache.count(r->n)) {
+                hbm_in_use_ += r->bytes;
+                held += r->bytes;
+            }
+        }
+        try {
+            auto &slot = cache[r->n];
+            if (!slot) slot = std::make_unique<StateVector>(r->n, cfg_.dtype);
+            run_engine(*slot, *r);
+        } catch (const std::exception &ex) {
+            r->res.error = ex.what();
+            r->res.path = 1;
+        }
+        complete(r);
+        cv_work_.notify_all();
+    }
+}
+
+} // namespace lqamd

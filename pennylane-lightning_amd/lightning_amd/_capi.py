@@ -68,6 +68,7 @@ _SIGS = {
     "lq_adjoint_jacobian": (C.c_int, [_P, _P, C.POINTER(_P), C.c_uint32, _U64P, C.c_uint32, C.c_int, _DP, _DP]),
     "lq_vector_jacobian_product": (C.c_int, [_P, _P, _P, C.c_uint64, _U64P, C.c_uint32, C.c_int, _DP]),
     "lq_sv_release_workspace": (C.c_int, [_P]),
+    "lq_sv_overlap": (C.c_int, [_P, _P, _DP, _DP]),
     "lq_sv_profile": (C.c_int, [_P, C.c_int]),
     "lq_sv_profile_read": (C.c_int, [_P, C.c_int, _U64P, _DP, _DP, _DP]),
     "lq_sv_stream": (C.c_int, [_P, C.POINTER(_P)]),

@@ -166,6 +166,13 @@ class _StateVectorBase:
     def release_workspace(self):
         check(lib.lq_sv_release_workspace(self._h))
 
+    def overlap(self, other) -> complex:
+        """<self|other> on the device, FP64 accumulation; the two states may
+        differ in precision."""
+        re, im = C.c_double(), C.c_double()
+        check(lib.lq_sv_overlap(self._h, other._h, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
     def pass_counts(self, reset: bool = True) -> dict:
         """Tile passes run as specialised kernels and as generic tile kernels."""
         a, b = C.c_uint64(), C.c_uint64()

@@ -1,0 +1,109 @@
+"""Request server: packed and engine paths against the oracle, each result the
+same alone and in a mix of concurrent requests, and the plain path against the
+batched one.
+
+Tolerances (complex128): energies and gradients 1e-10 against the oracle;
+alone vs in the mix 1e-12 (the same program runs either way; the engine path
+may run generic or specialised kernels, which round differently)."""
+import numpy as np
+import pytest
+
+from helpers import ham_to_oracle, oracle_run, to_oracle_ops
+from oracle import lq_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+S = pytest.importorskip("lightning_amd.serving")
+L = pytest.importorskip("lightning_amd.lightning_qubit_ops")
+
+
+def oracle_result(req):
+    st = oracle_run(req.n, req.ops)
+    H = ham_to_oracle(req.coeffs, req.words)
+    jac = O.adjoint_jacobian(st, req.n, to_oracle_ops(req.ops), [H], list(range(req.num_params)))
+    return O.expval_obs(st, req.n, H), jac[0]
+
+
+def small_requests(count, seed, n_lo=3, n_hi=12, bits=128):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(n_lo, n_hi + 1))
+        depth = int(rng.integers(1, 4))
+        r = np.random.default_rng([seed, i])
+        ops = S.hea_adjoint_ops(n, depth, r)
+        H = O.random_pauli_hamiltonian(n, int(rng.integers(3, 20)), seed=i, probs_ixyz=(0.4, 0.2, 0.2, 0.2))
+        out.append(S.Request(n, depth, ops, list(H[1]), [list(t[1]) for t in H[2]], bits))
+    return out
+
+
+@pytest.mark.parametrize("bits", [128, 64])
+@pytest.mark.parametrize("n_lo,n_hi", [(3, 12), (13, 16)])
+def test_packed_against_oracle(bits, n_lo, n_hi):
+    # up to 12 qubits (complex128) the states stay in LDS, above in global scratch
+    reqs = small_requests(12 if n_hi <= 12 else 5, seed=3, n_lo=n_lo, n_hi=n_hi, bits=bits)
+    srv = S.Server(bits=bits, batched=True)
+    res = srv.replay(reqs, np.zeros(len(reqs)))
+    st = srv.stats()
+    srv.close()
+    assert st["packed"] == len(reqs) and st["engine"] == 0
+    tol = 1e-10 if bits == 128 else 5e-4
+    for i, r in enumerate(reqs):
+        e, g = oracle_result(r)
+        assert abs(res["energies"][i] - e) < tol
+        np.testing.assert_allclose(res["grads"][i], g, atol=tol)
+
+
+def test_mixed_paths_alone_and_in_the_mix():
+    reqs = small_requests(8, seed=5, n_lo=4, n_hi=12)
+    reqs += small_requests(2, seed=6, n_lo=13, n_hi=16)
+    reqs += small_requests(3, seed=7, n_lo=17, n_hi=17)  # engine path
+    srv = S.Server(bits=128, batched=True, large_workers=2, batch_streams=2)
+    alone = [srv.run(r) for r in reqs]
+    mix = srv.replay(reqs, np.linspace(0.0, 0.002, len(reqs)))
+    st = srv.stats()
+    srv.close()
+    assert st["engine"] == 6 and st["packed"] == 20
+    for i, r in enumerate(reqs):
+        assert abs(mix["energies"][i] - alone[i][0]) < 1e-12
+        np.testing.assert_allclose(mix["grads"][i], alone[i][1], atol=1e-12)
+        e, g = oracle_result(r)
+        assert abs(alone[i][0] - e) < 1e-10
+        np.testing.assert_allclose(alone[i][1], g, atol=1e-10)
+    assert set(mix["path"][:10].tolist()) == {0} and set(mix["path"][10:].tolist()) == {1}
+
+
+def test_plain_matches_batched():
+    reqs = small_requests(8, seed=9, n_lo=5, n_hi=14)
+    plain = S.Server(bits=128, batched=False)
+    rp = plain.replay(reqs, np.zeros(len(reqs)))
+    plain.close()
+    batched = S.Server(bits=128, batched=True)
+    rb = batched.replay(reqs, np.zeros(len(reqs)))
+    batched.close()
+    assert set(rp["path"].tolist()) == {1}
+    np.testing.assert_allclose(rp["energies"], rb["energies"], atol=1e-10)
+    for a, b in zip(rp["grads"], rb["grads"]):
+        np.testing.assert_allclose(a, b, atol=1e-10)
+
+
+def test_serving_workload_sample():
+    """A sample of the serving workload (n in 10..16 here) through both paths."""
+    reqs = S.serving_requests(24, seed=2, n_max=16, depth_max=6)
+    srv = S.Server(bits=128, batched=True)
+    res = srv.replay(reqs, S.poisson_arrivals(len(reqs), qps=2000.0))
+    srv.close()
+    summ = S.summarize(res, len(reqs))
+    assert summ["p99_ms"] >= summ["p50_ms"] > 0
+    for i in (0, 5, 11):
+        e, g = oracle_result(reqs[i])
+        assert abs(res["energies"][i] - e) < 1e-10
+        np.testing.assert_allclose(res["grads"][i], g, atol=1e-10)
+
+
+def test_budget_rejects_oversized_request():
+    reqs = small_requests(1, seed=1, n_lo=12, n_hi=12)
+    srv = S.Server(bits=128, batched=True, hbm_budget=1 << 16)
+    with pytest.raises(L.LightningException):
+        srv.submit(reqs[0])
+    srv.close()
