@@ -201,8 +201,8 @@ def run_serving(args):
     setup_s = time.perf_counter() - t0
     warm = [r for r in reqs if r.n <= 14][:64]
     rows = []
-    qps_plain = [float(x) for x in (args.qps.split(",") if args.qps else ["100", "200", "400"])]
-    qps_batched = [float(x) for x in (args.qps.split(",") if args.qps else ["500", "1000", "2000", "4000"])]
+    qps_plain = [float(x) for x in (args.qps.split(",") if args.qps else ["20", "40", "60", "80"])]
+    qps_batched = [float(x) for x in (args.qps.split(",") if args.qps else ["40", "80", "120", "160", "240"])]
     for batched, sweep in ((False, qps_plain), (True, qps_batched)):
         srv = S.Server(bits=128, batched=batched)
         srv.replay(warm, np.zeros(len(warm)))
@@ -213,14 +213,20 @@ def run_serving(args):
             rows.append(summ)
             print(json.dumps({"serving_point": summ}), file=sys.stderr, flush=True)
         srv.close()
+    # a point keeps up with its arrivals when it completes at >= 90% of the
+    # offered rate; the plain reference point is its fastest such point
+    def keeps_up(r):
+        return r["req_per_s"] >= 0.9 * r["offered_qps"]
+
     plain = [r for r in rows if r["mode"] == "plain"]
-    best_plain = max(plain, key=lambda r: r["req_per_s"])
-    ok = [r for r in rows if r["mode"] == "batched" and r["p99_ms"] <= best_plain["p99_ms"]]
+    plain_ok = [r for r in plain if keeps_up(r)] or [min(plain, key=lambda r: r["p99_ms"])]
+    best_plain = max(plain_ok, key=lambda r: r["req_per_s"])
+    ok = [r for r in rows if r["mode"] == "batched" and keeps_up(r) and r["p99_ms"] <= best_plain["p99_ms"]]
     best_b = max(ok, key=lambda r: r["req_per_s"]) if ok else None
     ratio = best_b["req_per_s"] / best_plain["req_per_s"] if best_b else None
     out = {
         "metric": "adjoint-grad req/s at equal p99, batched vs plain GPU path",
-        "value": round(ratio, 3) if ratio else None, "unit": "x plain throughput at p99 <= plain p99",
+        "value": round(ratio, 3) if ratio else None, "unit": "x plain throughput (both keeping up with arrivals, batched p99 <= plain p99)",
         "n_gpus": 1, "steps": len(rows), "warmup": len(warm), "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
         "config": {"workload": f"{args.requests} requests, n 10..28 with P(n) ~ 2^-((n-10)/2), depth 4..16 HEA "
@@ -379,7 +385,7 @@ def main():
                          "serving: energy + gradient requests, batched vs plain; qft33: 33-qubit QFT + Haar "
                          "brickwork + 3-controlled X with complex64 fidelity; precision: complex64 error per "
                          "workload against the complex128 run")
-    ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
     args = ap.parse_args()
 

@@ -106,7 +106,37 @@ LOp identity_dot(uint32_t sa, uint32_t sb, int slot) {
     return make_dot(op, sa, sb, slot);
 }
 
-LOp pauli_op(const PauliWord &w) {
+// Pauli word as the cheapest op form for a dot: a parity op for Z-only
+// words, a dense 2x2 / 4x4 (or diagonal) op for words on one or two qubits,
+// the general Pauli-string op otherwise.
+LOp pauli_dot_op(int n, const PauliWord &w) {
+    (void)n;
+    const uint64_t sup = w.x | w.yz;
+    if (w.x == 0 && sup != 0) {
+        LOp op;
+        op.kind = TOP_PARITY;
+        op.pmask = w.yz;
+        op.coef = {1.0, -1.0};
+        return op;
+    }
+    const int k = __builtin_popcountll(sup);
+    if (k >= 1 && k <= 2) {
+        std::vector<int> tb;
+        std::vector<std::vector<cd>> fac;
+        for (int b = 63; b >= 0; --b) {
+            if (!((sup >> b) & 1ull)) continue;
+            tb.push_back(b); // descending bit = wires ascending: tb[0] is the local MSB
+            const bool x = (w.x >> b) & 1ull, z = (w.yz >> b) & 1ull;
+            fac.push_back(named_obs_matrix(x && z ? "PauliY" : x ? "PauliX" : "PauliZ"));
+        }
+        std::vector<cd> m = fac[0];
+        if (k == 2) {
+            m.assign(16, 0.0);
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) m[r * 4 + c] = fac[0][(r >> 1) * 2 + (c >> 1)] * fac[1][(r & 1) * 2 + (c & 1)];
+        }
+        return lower_matrix(std::move(m), tb, 0, 0, true);
+    }
     LOp op;
     op.kind = TOP_PAULI;
     op.xmask = w.x;
@@ -734,25 +764,30 @@ double StateVector::expval_named(const std::string &name, const uint32_t *wires,
     const auto &m = named_obs_matrix(name);
     LQ_ABORT_IF(nw != 1, "named observables act on one wire");
     check_wires(n_, wires, 1, nullptr, 0);
-    flush();
-    LOp op = make_dot(lower_matrix(m, {phys_bit(n_, wires[0])}, 0, 0, true), 0, 0, 0);
-    return run({op}, {d_}, 1)[0].real();
+    // the pending gates and the dot run as one program
+    std::vector<LOp> prog;
+    prog.swap(queue_);
+    prog.push_back(make_dot(lower_matrix(m, {phys_bit(n_, wires[0])}, 0, 0, true), 0, 0, 0));
+    return run(prog, {d_}, 1)[0].real();
 }
 
 std::vector<double> StateVector::expval_named_batch(const std::vector<std::string> &names,
                                                     const std::vector<uint32_t> &wires) {
     LQ_ABORT_IF(names.size() != wires.size(), "names and wires differ in length");
-    flush();
-    std::vector<LOp> prog;
+    std::vector<LOp> dots;
     std::vector<int> slot_of(names.size(), -1);
     for (size_t i = 0; i < names.size(); ++i) {
         if (names[i] == "Identity") continue;
         const auto &m = named_obs_matrix(names[i]);
         check_wires(n_, &wires[i], 1, nullptr, 0);
-        slot_of[i] = static_cast<int>(prog.size());
-        prog.push_back(make_dot(lower_matrix(m, {phys_bit(n_, wires[i])}, 0, 0, true), 0, 0, slot_of[i]));
+        slot_of[i] = static_cast<int>(dots.size());
+        dots.push_back(make_dot(lower_matrix(m, {phys_bit(n_, wires[i])}, 0, 0, true), 0, 0, slot_of[i]));
     }
-    const auto v = run(prog, {d_}, static_cast<int>(prog.size()));
+    const int nslots = static_cast<int>(dots.size());
+    std::vector<LOp> prog;
+    prog.swap(queue_);
+    for (auto &d : dots) prog.push_back(std::move(d));
+    const auto v = run(prog, {d_}, nslots);
     std::vector<double> out(names.size(), 1.0);
     for (size_t i = 0; i < names.size(); ++i)
         if (slot_of[i] >= 0) out[i] = v[slot_of[i]].real();
@@ -773,9 +808,10 @@ double StateVector::expval_matrix(const void *mat, const uint32_t *wires, int nw
     }
     std::vector<int> tb(nw);
     for (int j = 0; j < nw; ++j) tb[j] = phys_bit(n_, wires[j]);
-    flush();
-    LOp op = make_dot(lower_matrix(std::move(m), tb, 0, 0, true), 0, 0, 0);
-    return run({op}, {d_}, 1)[0].real();
+    std::vector<LOp> prog;
+    prog.swap(queue_);
+    prog.push_back(make_dot(lower_matrix(std::move(m), tb, 0, 0, true), 0, 0, 0));
+    return run(prog, {d_}, 1)[0].real();
 }
 
 cd StateVector::inner(const void *bra, const void *ket) {
@@ -862,16 +898,20 @@ void StateVector::apply_obs_to(const Obs &o, const void *in, void *out) {
 
 double StateVector::expval_obs(const Obs &o) {
     for (auto w : o.all_wires()) LQ_ABORT_IF(static_cast<int>(w) >= n_, "observable wire out of range");
-    flush();
     std::vector<PauliWord> words;
-    if (obs_to_pauli(o, n_, &words)) {
-        const int ktm = std::min(kt_for(1), n_);
-        bool fits = words.size() <= 32;
-        for (const auto &w : words) fits = fits && __builtin_popcountll(w.x) <= ktm - 2;
+    const bool pauli = obs_to_pauli(o, n_, &words);
+    const int ktm = std::min(kt_for(1), n_);
+    bool fits = pauli && words.size() <= 32;
+    for (const auto &w : words) fits = fits && __builtin_popcountll(w.x) <= ktm - 2;
+    if (!fits) flush();
+    if (pauli) {
         if (fits) {
+            // the pending gates and the dots run as one program
             std::vector<LOp> prog;
-            for (size_t i = 0; i < words.size(); ++i) prog.push_back(make_dot(pauli_op(words[i]), 0, 0, static_cast<int>(i)));
-            const auto v = run(prog, {d_}, static_cast<int>(prog.size()));
+            prog.swap(queue_);
+            for (size_t i = 0; i < words.size(); ++i)
+                prog.push_back(make_dot(pauli_dot_op(n_, words[i]), 0, 0, static_cast<int>(i)));
+            const auto v = run(prog, {d_}, static_cast<int>(words.size()));
             double e = 0.0;
             for (size_t i = 0; i < words.size(); ++i) e += (words[i].coeff * v[i]).real();
             return e;

@@ -56,6 +56,17 @@ __device__ __forceinline__ void acc(double &r, double &i, const cplx b, const cp
     i += bx * vy - by * vx;
 }
 __device__ __forceinline__ unsigned pc(unsigned long long x) { return (unsigned)__popcll(x); }
+typedef real nvec __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cplx ldnt(const cplx *p) {
+    const nvec t = __builtin_nontemporal_load(reinterpret_cast<const nvec *>(p));
+    return mk(t.x, t.y);
+}
+__device__ __forceinline__ void stnt(cplx *p, const cplx v) {
+    nvec t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<nvec *>(p));
+}
 __device__ __forceinline__ unsigned swz(unsigned l) { return LQ_SWZ; }
 __device__ __forceinline__ void dflush(double r, double i, double2 (*dred)[32], unsigned slot) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -99,7 +110,7 @@ struct Mapping {
 
 class Gen {
   public:
-    Gen(const PassPlan &p, int n, int dtype, int ns) : p_(p), dtype_(dtype), ns_(ns) {
+    Gen(const PassPlan &p, int n, int dtype, int ns) : p_(p), dtype_(dtype), ns_(ns), nt_(jit_config().nontemporal) {
         kt_ = static_cast<int>(p.tbits.size());
         rb_ = kt_ - 8;
         E_ = 1 << rb_;
@@ -136,8 +147,12 @@ class Gen {
         emit_gl("gl0", m0);
         for (int s = 0; s < ns_; ++s) {
             if (!((p_.load_mask >> s) & 1u)) continue;
-            for (int e = 0; e < E_; ++e)
-                o << "  v" << s << "[" << e << "] = sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "];\n";
+            for (int e = 0; e < E_; ++e) {
+                if (nt_)
+                    o << "  v" << s << "[" << e << "] = ldnt(&sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "]);\n";
+                else
+                    o << "  v" << s << "[" << e << "] = sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "];\n";
+            }
         }
         for (size_t si = 0; si < p_.stages.size(); ++si) {
             const TileStage &st = p_.stages[si];
@@ -156,8 +171,12 @@ class Gen {
             cur_ = m0;
             for (int s = 0; s < ns_; ++s) {
                 if (!((p_.store_mask >> s) & 1u)) continue;
-                for (int e = 0; e < E_; ++e)
-                    o << "  sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "] = v" << s << "[" << e << "];\n";
+                for (int e = 0; e < E_; ++e) {
+                    if (nt_)
+                        o << "  stnt(&sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "], v" << s << "[" << e << "]);\n";
+                    else
+                        o << "  sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "] = v" << s << "[" << e << "];\n";
+                }
             }
         }
         if (nslots) {
@@ -174,7 +193,9 @@ class Gen {
 
   private:
     const PassPlan &p_;
-    int dtype_, ns_, kt_, rb_, E_;
+    int dtype_, ns_;
+    bool nt_;
+    int kt_, rb_, E_;
     uint32_t tsize_;
     TileGeom geom_;
     Mapping cur_{};
@@ -487,6 +508,7 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_SYNC")) j.sync = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_MIN_QUBITS")) j.min_qubits = std::atoi(s);
         if (const char *s = std::getenv("LQ_JIT_THREADS")) j.threads = std::max(1, std::atoi(s));
+        if (const char *s = std::getenv("LQ_JIT_NT")) j.nontemporal = std::atoi(s) != 0;
         return j;
     }();
     return c;

@@ -24,6 +24,7 @@ struct JitConfig {
     bool sync = false;      // compile before first use instead of in the background
     int min_qubits = 16;    // smaller states always use the generic kernels
     int threads = 8;        // background compile threads
+    bool nontemporal = false; // state loads/stores bypass the caches' normal retention
 };
 JitConfig &jit_config();
 
