@@ -28,8 +28,11 @@ const char *kPrelude = R"(
 typedef LQ_REAL real;
 typedef LQ_CPLX cplx;
 __device__ __forceinline__ cplx mk(real x, real y) { cplx r; r.x = x; r.y = y; return r; }
+// coefficients are read through the constant address space: they do not
+// change while a pass runs, so these are scalar loads whatever the kernel's size
+typedef __attribute__((address_space(4))) const double2 kcoef;
 __device__ __forceinline__ cplx ldc(const double2 *__restrict__ c, int i) {
-    const double2 d = c[i];
+    const double2 d = ((const kcoef *)c)[i];
     return mk((real)d.x, (real)d.y);
 }
 __device__ __forceinline__ cplx cm(const cplx a, const cplx b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
@@ -55,7 +58,39 @@ __device__ __forceinline__ void acc(double &r, double &i, const cplx b, const cp
     r += bx * vx + by * vy;
     i += bx * vy - by * vx;
 }
+// predicated forms for ops whose controls depend on the thread: computed by
+// every lane and selected, so the wave never diverges
+__device__ __forceinline__ cplx sel(const bool g, const cplx a, const cplx b) { return mk(g ? a.x : b.x, g ? a.y : b.y); }
+__device__ __forceinline__ void d1p(cplx &a, cplx &b, const cplx m0, const cplx m1, const cplx m2, const cplx m3,
+                                    const bool g) {
+    cplx x = a, y = b;
+    d1(x, y, m0, m1, m2, m3);
+    a = sel(g, x, a);
+    b = sel(g, y, b);
+}
+__device__ __forceinline__ void d2p(cplx &a, cplx &b, cplx &c, cplx &d, const cplx *m, const bool g) {
+    cplx w = a, x = b, y = c, z = d;
+    d2(w, x, y, z, m);
+    a = sel(g, w, a);
+    b = sel(g, x, b);
+    c = sel(g, y, c);
+    d = sel(g, z, d);
+}
+__device__ __forceinline__ void accp(double &r, double &i, const cplx b, const cplx v, const bool g) {
+    double tr = 0.0, ti = 0.0;
+    acc(tr, ti, b, v);
+    r += g ? tr : 0.0;
+    i += g ? ti : 0.0;
+}
 __device__ __forceinline__ unsigned pc(unsigned long long x) { return (unsigned)__popcll(x); }
+// a zero the compiler cannot see through: coefficient offsets depend on it,
+// so coefficient loads stay next to their use in each tile instead of being
+// hoisted out of the tile loop (and kept in registers across it)
+__device__ __forceinline__ unsigned opaque0() {
+    unsigned z = 0;
+    asm volatile("" : "+s"(z));
+    return z;
+}
 typedef real nvec __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ cplx ldnt(const cplx *p) {
     const nvec t = __builtin_nontemporal_load(reinterpret_cast<const nvec *>(p));
@@ -110,7 +145,8 @@ struct Mapping {
 
 class Gen {
   public:
-    Gen(const PassPlan &p, int n, int dtype, int ns) : p_(p), dtype_(dtype), ns_(ns), nt_(jit_config().nontemporal) {
+    Gen(const PassPlan &p, int n, int dtype, int ns)
+        : p_(p), dtype_(dtype), ns_(ns), nt_(jit_config().nontemporal), persist_(jit_persistent(dtype)) {
         kt_ = static_cast<int>(p.tbits.size());
         rb_ = kt_ - 8;
         E_ = 1 << rb_;
@@ -131,29 +167,91 @@ class Gen {
         const int nslots = static_cast<int>(p_.slot_ids.size());
         if (nslots) o << "  __shared__ double2 dred[4][32];\n";
         o << "  const unsigned tid = threadIdx.x;\n";
-        o << "  const unsigned long long tile = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * gridDim.x;\n";
-        o << "  if (tile >= ntiles) return;\n";
+        Mapping m0{};
+        for (int i = 0; i < rb_; ++i) m0.r[i] = 8 + i;
+        for (int j = 0; j < 8; ++j) m0.h[j] = j;
+        for (int s = 0; s < ns_; ++s) o << "  cplx v" << s << "[" << E_ << "];\n";
+        if (!persist_) {
+            o << "  const unsigned long long tile = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * gridDim.x;\n";
+            o << "  if (tile >= ntiles) return;\n";
+            o << "  {\n";
+            emit_base_gl("tile", "gl0", m0);
+            emit_loads("v", "gl0", m0);
+            o << "  }\n";
+            emit_tile_body(m0, nslots);
+        } else {
+            // persistent workgroups: each walks tiles blockIdx.x + k * gridDim.x
+            // and loads the next tile while it works on the current one
+            for (int s = 0; s < ns_; ++s) o << "  cplx n" << s << "[" << E_ << "];\n";
+            o << "  unsigned long long tile = blockIdx.x;\n";
+            o << "  if (tile >= ntiles) return;\n";
+            o << "  {\n";
+            emit_base_gl("tile", "gl0", m0);
+            emit_loads("v", "gl0", m0);
+            o << "  }\n";
+            o << "  while (true) {\n";
+            o << "  const unsigned long long ntile = tile + gridDim.x;\n";
+            o << "  const bool more = ntile < ntiles;\n";
+            // unconditional (the last tile loads itself again), so the loads
+            // stay in flight across the work on the current tile
+            o << "  const unsigned long long ltile = more ? ntile : tile;\n";
+            o << "  {\n";
+            emit_base_gl("ltile", "gln", m0);
+            emit_loads("n", "gln", m0);
+            o << "  }\n";
+            emit_tile_body(m0, nslots);
+            o << "  if (!more) break;\n";
+            o << "  tile = ntile;\n";
+            for (int s = 0; s < ns_; ++s) {
+                if (!((p_.load_mask >> s) & 1u)) continue;
+                for (int e = 0; e < E_; ++e) o << "  v" << s << "[" << e << "] = n" << s << "[" << e << "];\n";
+            }
+            o << "  __syncthreads();\n";
+            lds_used_ = false;
+            o << "  }\n";
+        }
+        o << "}\n";
+        return o.str();
+    }
+
+  private:
+    const PassPlan &p_;
+    int dtype_, ns_;
+    bool nt_;
+    bool persist_;
+
+    void emit_base_gl(const char *tile, const char *gl, const Mapping &m) {
+        o_ << "  const unsigned long long " << (std::string(gl) == "gl0" ? "b0" : "bn") << " = "
+           << deposit(tile, geom_.ob, geom_.nout, true) << ";\n";
+        int pos[8];
+        for (int j = 0; j < 8; ++j) pos[j] = geom_.tb[m.h[j]];
+        o_ << "  const unsigned long long " << gl << "_ = " << (std::string(gl) == "gl0" ? "b0" : "bn") << " | "
+           << deposit("tid", pos, 8, true) << ";\n";
+    }
+    void emit_loads(const char *dst, const char *gl, const Mapping &m) {
+        for (int s = 0; s < ns_; ++s) {
+            if (!((p_.load_mask >> s) & 1u)) continue;
+            for (int e = 0; e < E_; ++e) {
+                const std::string addr = std::string("&sv") + std::to_string(s) + "[" + gl + "_ | " + hex64(elem_global(m, e)) + "]";
+                if (nt_)
+                    o_ << "  " << dst << s << "[" << e << "] = ldnt(" << addr << ");\n";
+                else
+                    o_ << "  " << dst << s << "[" << e << "] = *(" << addr << ");\n";
+            }
+        }
+    }
+    // stages, store and dot partials of the tile `tile` held in v*
+    void emit_tile_body(const Mapping &m0, int nslots) {
+        std::ostringstream &o = o_;
+        o << "  {\n";
         o << "  const unsigned long long base = " << deposit("tile", geom_.ob, geom_.nout, true) << ";\n";
+        o << "  const unsigned cz = opaque0();\n";
         if (nslots) {
             o << "  for (unsigned i = tid; i < 128u; i += 256u) dred[i >> 5][i & 31u] = make_double2(0.0, 0.0);\n";
             o << "  __syncthreads();\n";
         }
-        for (int s = 0; s < ns_; ++s) o << "  cplx v" << s << "[" << E_ << "];\n";
-        // load mapping: thread bits -> tile bits 0..7, register bits -> tile bits 8..
-        Mapping m0{};
-        for (int i = 0; i < rb_; ++i) m0.r[i] = 8 + i;
-        for (int j = 0; j < 8; ++j) m0.h[j] = j;
         cur_ = m0;
         emit_gl("gl0", m0);
-        for (int s = 0; s < ns_; ++s) {
-            if (!((p_.load_mask >> s) & 1u)) continue;
-            for (int e = 0; e < E_; ++e) {
-                if (nt_)
-                    o << "  v" << s << "[" << e << "] = ldnt(&sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "]);\n";
-                else
-                    o << "  v" << s << "[" << e << "] = sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "];\n";
-            }
-        }
         for (size_t si = 0; si < p_.stages.size(); ++si) {
             const TileStage &st = p_.stages[si];
             Mapping m{};
@@ -187,14 +285,8 @@ class Gen {
             o << "    partials[(unsigned long long)tid * ntiles + tile] = t;\n";
             o << "  }\n";
         }
-        o << "}\n";
-        return o.str();
+        o << "  }\n";
     }
-
-  private:
-    const PassPlan &p_;
-    int dtype_, ns_;
-    bool nt_;
     int kt_, rb_, E_;
     uint32_t tsize_;
     TileGeom geom_;
@@ -283,65 +375,69 @@ class Gen {
         const std::string A = "v" + std::to_string(t.sa), B = "v" + std::to_string(t.sb);
         o << "  { // op " << id << " kind " << t.kind << "\n";
         if (dot) o << "    double ar = 0.0, ai = 0.0;\n";
-        const std::string open = guard.empty() ? "" : "    if (" + guard + ") {\n";
-        const std::string close = guard.empty() ? "" : "    }\n";
+        const bool pred = !guard.empty();
+        if (pred) o << "    const bool gd = " << guard << ";\n";
+        const std::string accf = pred ? "accp(" : "acc(";
+        const std::string gsuf = pred ? ", gd)" : ")";
         const uint64_t off = t.coef_off;
         switch (t.kind) {
         case TOP_DENSE1: {
             const int I = t.lpos[0];
-            o << "    const cplx m0 = ldc(coef, " << off << "), m1 = ldc(coef, " << off + 1 << "), m2 = ldc(coef, " << off + 2
-              << "), m3 = ldc(coef, " << off + 3 << ");\n";
-            o << open;
+            o << "    const cplx m0 = ldc(coef, cz + " << off << "), m1 = ldc(coef, cz + " << off + 1 << "), m2 = ldc(coef, cz + " << off + 2
+              << "), m3 = ldc(coef, cz + " << off + 3 << ");\n";
             for (int e = 0; e < E_; ++e) {
                 if ((e >> I) & 1) continue;
                 if (!elem_ok(t.gcmask, t.gcval, e)) continue;
                 const int e1 = e | (1 << I);
                 if (dot) {
-                    o << "    acc(ar, ai, " << B << "[" << e << "], cm2(m0, " << A << "[" << e << "], m1, " << A << "[" << e1
-                      << "]));\n";
-                    o << "    acc(ar, ai, " << B << "[" << e1 << "], cm2(m2, " << A << "[" << e << "], m3, " << A << "["
-                      << e1 << "]));\n";
+                    o << "    " << accf << "ar, ai, " << B << "[" << e << "], cm2(m0, " << A << "[" << e << "], m1, " << A
+                      << "[" << e1 << "])" << gsuf << ";\n";
+                    o << "    " << accf << "ar, ai, " << B << "[" << e1 << "], cm2(m2, " << A << "[" << e << "], m3, " << A
+                      << "[" << e1 << "])" << gsuf << ";\n";
                 } else {
-                    for (int s : sts) o << "    d1(v" << s << "[" << e << "], v" << s << "[" << e1 << "], m0, m1, m2, m3);\n";
+                    for (int s : sts)
+                        o << "    " << (pred ? "d1p(" : "d1(") << "v" << s << "[" << e << "], v" << s << "[" << e1
+                          << "], m0, m1, m2, m3" << gsuf << ";\n";
                 }
             }
-            o << close;
             break;
         }
         case TOP_DENSE2: {
             const int I0 = t.lpos[0], I1 = t.lpos[1];
             o << "    cplx M[16];\n";
-            for (int i = 0; i < 16; ++i) o << "    M[" << i << "] = ldc(coef, " << off + i << ");\n";
-            o << open;
+            for (int i = 0; i < 16; ++i) o << "    M[" << i << "] = ldc(coef, cz + " << off + i << ");\n";
             for (int e = 0; e < E_; ++e) {
                 if (((e >> I0) & 1) || ((e >> I1) & 1)) continue;
                 if (!elem_ok(t.gcmask, t.gcval, e)) continue;
                 const int idx[4] = {e, e | (1 << I1), e | (1 << I0), e | (1 << I0) | (1 << I1)};
                 if (dot) {
                     for (int r = 0; r < 4; ++r) {
-                        o << "    acc(ar, ai, " << B << "[" << idx[r] << "], add(cm2(M[" << 4 * r << "], " << A << "["
-                          << idx[0] << "], M[" << 4 * r + 1 << "], " << A << "[" << idx[1] << "]), cm2(M[" << 4 * r + 2
-                          << "], " << A << "[" << idx[2] << "], M[" << 4 * r + 3 << "], " << A << "[" << idx[3] << "])));\n";
+                        o << "    " << accf << "ar, ai, " << B << "[" << idx[r] << "], add(cm2(M[" << 4 * r << "], " << A
+                          << "[" << idx[0] << "], M[" << 4 * r + 1 << "], " << A << "[" << idx[1] << "]), cm2(M["
+                          << 4 * r + 2 << "], " << A << "[" << idx[2] << "], M[" << 4 * r + 3 << "], " << A << "["
+                          << idx[3] << "]))" << gsuf << ";\n";
                     }
                 } else {
                     for (int s : sts)
-                        o << "    d2(v" << s << "[" << idx[0] << "], v" << s << "[" << idx[1] << "], v" << s << "["
-                          << idx[2] << "], v" << s << "[" << idx[3] << "], M);\n";
+                        o << "    " << (pred ? "d2p(" : "d2(") << "v" << s << "[" << idx[0] << "], v" << s << "["
+                          << idx[1] << "], v" << s << "[" << idx[2] << "], v" << s << "[" << idx[3] << "], M" << gsuf
+                          << ";\n";
                 }
             }
-            o << close;
             break;
         }
         case TOP_DIAG:
         case TOP_PCPHASE:
-        case TOP_PARITY: emit_phase(t, sts, dot, A, B, open, close); break;
+        case TOP_PARITY: emit_phase(t, sts, dot, A, B, pred); break;
         case TOP_SIGN: {
-            o << open;
             for (int e = 0; e < E_; ++e) {
                 if (!elem_ok(t.gcmask, t.gcval, e)) continue;
-                for (int s : sts) o << "    v" << s << "[" << e << "] = ng(v" << s << "[" << e << "]);\n";
+                for (int s : sts) {
+                    const std::string v = "v" + std::to_string(s) + "[" + std::to_string(e) + "]";
+                    if (pred) o << "    " << v << " = sel(gd, ng(" << v << "), " << v << ");\n";
+                    else o << "    " << v << " = ng(" << v << ");\n";
+                }
             }
-            o << close;
             break;
         }
         case TOP_CZSET: emit_czset(t, sts); break;
@@ -354,14 +450,14 @@ class Gen {
     // DIAG (k <= 2 in register passes), PCPHASE, PARITY: a phase per element
     // chosen by bits of its global index.
     void emit_phase(const TileOp &t, const std::vector<int> &sts, bool dot, const std::string &A,
-                    const std::string &B, const std::string &open, const std::string &close) {
+                    const std::string &B, bool pred) {
         std::ostringstream &o = o_;
         const uint64_t off = t.coef_off;
         // per element: phase index = thread part (runtime) | register part (constant)
         std::vector<std::string> phase_of(E_);
         if (t.kind == TOP_PARITY) {
             const uint64_t tm = t.gpmask & ~regmask();
-            o << "    const cplx d0 = ldc(coef, " << off << "), d1_ = ldc(coef, " << off + 1 << ");\n";
+            o << "    const cplx d0 = ldc(coef, cz + " << off << "), d1_ = ldc(coef, cz + " << off + 1 << ");\n";
             if (tm) {
                 o << "    const bool pt = (pc(gt & " << hex64(tm) << ") & 1u) != 0u;\n";
                 o << "    const cplx q0 = pt ? d1_ : d0, q1 = pt ? d0 : d1_;\n";
@@ -390,7 +486,7 @@ class Gen {
             }
             o << "    const unsigned ti = " << (any ? ti.str() : "0u") << ";\n";
             const int nd = t.kind == TOP_PCPHASE ? 2 : (1 << k);
-            for (int i = 0; i < nd; ++i) o << "    const cplx d" << i << " = ldc(coef, " << off + i << ");\n";
+            for (int i = 0; i < nd; ++i) o << "    const cplx d" << i << " = ldc(coef, cz + " << off + i << ");\n";
             std::vector<int> done(1 << std::max(k, 1), 0);
             for (int e = 0; e < E_; ++e) {
                 uint32_t rpart = 0;
@@ -413,17 +509,19 @@ class Gen {
                 phase_of[e] = name;
             }
         }
-        o << open;
         for (int e = 0; e < E_; ++e) {
             if (!elem_ok(t.gcmask, t.gcval, e)) continue;
             if (dot) {
-                o << "    acc(ar, ai, " << B << "[" << e << "], cm(" << phase_of[e] << ", " << A << "[" << e << "]));\n";
+                o << "    " << (pred ? "accp(" : "acc(") << "ar, ai, " << B << "[" << e << "], cm(" << phase_of[e] << ", "
+                  << A << "[" << e << "])" << (pred ? ", gd)" : ")") << ";\n";
             } else {
-                for (int s : sts)
-                    o << "    v" << s << "[" << e << "] = cm(" << phase_of[e] << ", v" << s << "[" << e << "]);\n";
+                for (int s : sts) {
+                    const std::string v = "v" + std::to_string(s) + "[" + std::to_string(e) + "]";
+                    if (pred) o << "    " << v << " = sel(gd, cm(" << phase_of[e] << ", " << v << "), " << v << ");\n";
+                    else o << "    " << v << " = cm(" << phase_of[e] << ", " << v << ");\n";
+                }
             }
         }
-        o << close;
     }
 
     // CZ runs: negate where the number of set (a, a + d) pairs is odd. Pairs
@@ -501,6 +599,13 @@ std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns) {
 // ---------------------------------------------------------------------------
 // Compile service
 // ---------------------------------------------------------------------------
+bool jit_persistent(int dtype) {
+    // complex64 kernels stay one workgroup per tile: a persistent complex64
+    // pass with dot products stopped making progress on the GPU once, cause
+    // not found, so that form is off until it is
+    return jit_config().persistent && dtype == 1;
+}
+
 JitConfig &jit_config() {
     static JitConfig c = [] {
         JitConfig j;
@@ -509,6 +614,7 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_MIN_QUBITS")) j.min_qubits = std::atoi(s);
         if (const char *s = std::getenv("LQ_JIT_THREADS")) j.threads = std::max(1, std::atoi(s));
         if (const char *s = std::getenv("LQ_JIT_NT")) j.nontemporal = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_JIT_PERSIST")) j.persistent = std::atoi(s) != 0;
         return j;
     }();
     return c;
@@ -727,14 +833,41 @@ void jit_wait() { Service::get().wait(); }
 void jit_shutdown() { Service::get().stop(); }
 void jit_stats(int *ready, int *failed, int *pending) { Service::get().stats(ready, failed, pending); }
 
-void jit_launch(hipFunction_t f, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
+void jit_launch(hipFunction_t f, int dtype, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
                 size_t lds_bytes, hipStream_t s) {
     void *a0 = sv0, *a1 = sv1, *a3 = partials;
     const void *a2 = coef;
     unsigned long long nt = ntiles;
     void *args[] = {&a0, &a1, &a2, &a3, &nt};
-    const uint32_t gx = static_cast<uint32_t>(ntiles < 65536 ? ntiles : 65536);
-    const uint32_t gy = static_cast<uint32_t>((ntiles + gx - 1) / gx);
+    uint32_t gx, gy = 1;
+    if (jit_persistent(dtype)) {
+        // one resident wave of workgroups; each loops over its tiles
+        static std::mutex mu;
+        static std::unordered_map<hipFunction_t, uint32_t> resident;
+        uint32_t slots = 0;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = resident.find(f);
+            if (it != resident.end()) slots = it->second;
+        }
+        if (!slots) {
+            int per_cu = 0, cus = 0, dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, lds_bytes) != hipSuccess ||
+                per_cu < 1 || cus < 1) {
+                per_cu = 1;
+                cus = 256;
+            }
+            slots = static_cast<uint32_t>(per_cu * cus);
+            std::lock_guard<std::mutex> lk(mu);
+            resident[f] = slots;
+        }
+        gx = static_cast<uint32_t>(ntiles < slots ? ntiles : slots);
+    } else {
+        gx = static_cast<uint32_t>(ntiles < 65536 ? ntiles : 65536);
+        gy = static_cast<uint32_t>((ntiles + gx - 1) / gx);
+    }
     const hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, kBlock, 1, 1, static_cast<unsigned>(lds_bytes), s, args, nullptr);
     if (e != hipSuccess) {
         const std::string m = std::string("hipModuleLaunchKernel: ") + hipGetErrorString(e);

@@ -24,7 +24,9 @@ struct JitConfig {
     bool sync = false;      // compile before first use instead of in the background
     int min_qubits = 16;    // smaller states always use the generic kernels
     int threads = 8;        // background compile threads
-    bool nontemporal = false; // state loads/stores bypass the caches' normal retention
+    bool nontemporal = true;  // state loads/stores marked nontemporal (streamed, not retained)
+    bool persistent = false;  // workgroups loop over tiles, loading the next while computing
+                              // (complex128 only; measured slower than one workgroup per tile)
 };
 JitConfig &jit_config();
 
@@ -42,8 +44,10 @@ void jit_shutdown();
 // Counters: kernels ready, compiles failed, compiles pending.
 void jit_stats(int *ready, int *failed, int *pending);
 
+// Whether kernels of this precision are generated in the persistent form.
+bool jit_persistent(int dtype);
 // Launch a specialised pass (arguments as the generic register kernel).
-void jit_launch(hipFunction_t f, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
+void jit_launch(hipFunction_t f, int dtype, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
                 size_t lds_bytes, hipStream_t s);
 
 } // namespace lqamd
