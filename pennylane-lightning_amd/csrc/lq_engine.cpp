@@ -397,7 +397,17 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
         cfg.reg_rb = ktr - 8;
         cfg.max_dots = kMaxPassDots;
     }
-    const std::vector<PassPlan> passes = plan_passes(prog, cfg);
+    // register passes that will be specialised: 2x2 gates as phase / real
+    // rotation / phase, so the rotations take real arithmetic and a stage's
+    // phases apply as one table
+    const JitConfig &jcfg = jit_config();
+    std::vector<LOp> split;
+    const bool do_split = cfg.reg_rb > 0 && jcfg.enabled && n_ >= jcfg.min_qubits && jcfg.split_rotations;
+    if (do_split) {
+        split = prog;
+        split_1q_rotations(split);
+    }
+    const std::vector<PassPlan> passes = plan_passes(do_split ? split : prog, cfg);
     if (t.plan_debug) {
         for (size_t p = 0; p < passes.size(); ++p) {
             const PassPlan &pp = passes[p];
@@ -461,7 +471,7 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
 
     // specialised kernels for register passes of large states
     std::vector<hipFunction_t> fns(passes.size(), nullptr);
-    const JitConfig &jc = jit_config();
+    const JitConfig &jc = jcfg;
     if (jc.enabled && n_ >= jc.min_qubits) {
         for (size_t p = 0; p < passes.size(); ++p) {
             if (!passes[p].reg) continue;

@@ -58,6 +58,13 @@ __device__ __forceinline__ void acc(double &r, double &i, const cplx b, const cp
     r += bx * vx + by * vy;
     i += bx * vy - by * vx;
 }
+// 2x2 with real coefficients: half the multiply-adds of the complex form
+__device__ __forceinline__ void d1r(cplx &a, cplx &b, const real m0, const real m1, const real m2, const real m3) {
+    const cplx x = a, y = b;
+    a = mk(m0 * x.x + m1 * y.x, m0 * x.y + m1 * y.y);
+    b = mk(m2 * x.x + m3 * y.x, m2 * x.y + m3 * y.y);
+}
+__device__ __forceinline__ real ldr(const double2 *__restrict__ c, int i) { return (real)((const kcoef *)c)[i].x; }
 // predicated forms for ops whose controls depend on the thread: computed by
 // every lane and selected, so the wave never diverges
 __device__ __forceinline__ cplx sel(const bool g, const cplx a, const cplx b) { return mk(g ? a.x : b.x, g ? a.y : b.y); }
@@ -383,6 +390,16 @@ class Gen {
         switch (t.kind) {
         case TOP_DENSE1: {
             const int I = t.lpos[0];
+            if ((t.flags & TF_REAL) && !dot && !pred) {
+                o << "    const real r0 = ldr(coef, cz + " << off << "), r1 = ldr(coef, cz + " << off + 1
+                  << "), r2 = ldr(coef, cz + " << off + 2 << "), r3 = ldr(coef, cz + " << off + 3 << ");\n";
+                for (int e = 0; e < E_; ++e) {
+                    if ((e >> I) & 1) continue;
+                    const int e1 = e | (1 << I);
+                    for (int s : sts) o << "    d1r(v" << s << "[" << e << "], v" << s << "[" << e1 << "], r0, r1, r2, r3);\n";
+                }
+                break;
+            }
             o << "    const cplx m0 = ldc(coef, cz + " << off << "), m1 = ldc(coef, cz + " << off + 1 << "), m2 = ldc(coef, cz + " << off + 2
               << "), m3 = ldc(coef, cz + " << off + 3 << ");\n";
             for (int e = 0; e < E_; ++e) {
@@ -441,6 +458,12 @@ class Gen {
             break;
         }
         case TOP_CZSET: emit_czset(t, sts); break;
+        case TOP_RPHASE: {
+            for (int e = 0; e < E_; ++e)
+                for (int s : sts)
+                    o << "    v" << s << "[" << e << "] = cm(ldc(coef, cz + " << off + e << "), v" << s << "[" << e << "]);\n";
+            break;
+        }
         default: LQ_ABORT("kernel generator: unsupported op");
         }
         if (dot) o << "    dflush(ar, ai, dred, " << t.slot << "u);\n";
@@ -581,7 +604,8 @@ std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns) {
         case TOP_DENSE2:
         case TOP_PARITY:
         case TOP_SIGN:
-        case TOP_CZSET: break;
+        case TOP_CZSET:
+        case TOP_RPHASE: break;
         case TOP_DIAG:
             if (t.k > 2) return "";
             break;
@@ -615,6 +639,7 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_THREADS")) j.threads = std::max(1, std::atoi(s));
         if (const char *s = std::getenv("LQ_JIT_NT")) j.nontemporal = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_PERSIST")) j.persistent = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_JIT_SPLIT")) j.split_rotations = std::atoi(s) != 0;
         return j;
     }();
     return c;

@@ -25,6 +25,7 @@ struct JitConfig {
     int min_qubits = 16;    // smaller states always use the generic kernels
     int threads = 8;        // background compile threads
     bool nontemporal = true;  // state loads/stores marked nontemporal (streamed, not retained)
+    bool split_rotations = true; // 2x2 gates as phase / real rotation / phase in specialised passes
     bool persistent = false;  // workgroups loop over tiles, loading the next while computing
                               // (complex128 only; measured slower than one workgroup per tile)
 };

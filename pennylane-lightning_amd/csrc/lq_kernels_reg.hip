@@ -171,6 +171,18 @@ __device__ __forceinline__ void reg_diag(Regs<T, NS, RB> &R, const TileOp &op, c
         }
         return;
     }
+    if (kind == TOP_RPHASE) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const V f = ld_coef<T>(coef, op.coef_off + e);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (!((op.smask >> s) & 1u)) continue;
+                R.v[s][e] = cmul(f, R.v[s][e]);
+            }
+        }
+        return;
+    }
     const int k = static_cast<int>(op.k);
     V d[4];
     const int nd = kind == TOP_DIAG ? (1 << k) : 2;

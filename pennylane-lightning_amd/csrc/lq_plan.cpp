@@ -71,10 +71,10 @@ uint64_t LOp::dmask() const {
 double LOp::cost() const {
     double c = 0.5;
     switch (kind) {
-    case TOP_DENSE1: c = 1.0; break;
+    case TOP_DENSE1: c = (flags & TF_REAL) ? 0.5 : 1.0; break; // real rotations take half the arithmetic
     case TOP_DENSE2: c = 2.5; break;
     case TOP_DENSEK: c = static_cast<double>(1 << k) / 2.0; break;
-    case TOP_DIAG: c = 0.5; break;
+    case TOP_DIAG: c = (k == 1 && cmask == 0) ? 0.15 : 0.5; break; // one-bit phases share a table per stage
     case TOP_PARITY: c = 0.4; break;
     case TOP_PCPHASE: c = 0.5; break;
     case TOP_PAULI: c = 0.6; break;
@@ -254,6 +254,80 @@ std::vector<cd> as_2x2(const LOp &o) {
 }
 } // namespace
 
+void split_1q_rotations(std::vector<LOp> &prog) {
+    std::vector<LOp> out;
+    out.reserve(prog.size() * 3);
+    for (auto &o : prog) {
+        const bool candidate = (o.kind == TOP_DENSE1 || o.kind == TOP_DENSEK) && o.k == 1 && o.cmask == 0 &&
+                               !o.is_dot() && !(o.flags & TF_ZERO_UNCTRL) && o.coef.size() == 4;
+        if (!candidate) {
+            out.push_back(std::move(o));
+            continue;
+        }
+        const cd u00 = o.coef[0], u01 = o.coef[1], u10 = o.coef[2], u11 = o.coef[3];
+        const double gamma = std::arg(u00 * u11 - u01 * u10) / 2.0;
+        const cd eg = std::polar(1.0, -gamma);
+        const cd a = eg * u00, b = eg * u10;
+        const double c = std::abs(a), sn = std::abs(b);
+        const double pa = c > 1e-300 ? std::arg(a) : 0.0, pb = sn > 1e-300 ? std::arg(b) : 0.0;
+        const double alpha = pb - pa, delta = -pa - pb;
+        const cd d1_0 = std::polar(1.0, -delta / 2), d1_1 = std::polar(1.0, delta / 2);
+        const cd d2_0 = std::polar(1.0, gamma - alpha / 2), d2_1 = std::polar(1.0, gamma + alpha / 2);
+        // U' = D2 R D1 with R = [[c, -s], [s, c]]
+        const cd r00 = d2_0 * c * d1_0, r01 = d2_0 * (-sn) * d1_1, r10 = d2_1 * sn * d1_0, r11 = d2_1 * c * d1_1;
+        const double err = std::max(std::max(std::abs(r00 - u00), std::abs(r01 - u01)),
+                                    std::max(std::abs(r10 - u10), std::abs(r11 - u11)));
+        if (!(err <= 1e-13)) {
+            out.push_back(std::move(o));
+            continue;
+        }
+        LOp p1;
+        p1.kind = TOP_DIAG;
+        p1.k = 1;
+        p1.bits[0] = o.bits[0];
+        p1.coef = {d1_0, d1_1};
+        p1.smask = o.smask;
+        LOp rot = o;
+        rot.kind = TOP_DENSE1;
+        rot.flags |= TF_REAL;
+        rot.coef = {cd(c), cd(-sn), cd(sn), cd(c)};
+        LOp p2 = p1;
+        p2.coef = {d2_0, d2_1};
+        out.push_back(std::move(p1));
+        out.push_back(std::move(rot));
+        out.push_back(std::move(p2));
+    }
+    // Move each one-bit phase later past the ops it commutes with (anything
+    // acting diagonally on its bit or on other states) and fold it into the
+    // next one-bit phase on the same bit, so that consecutive rotations on a
+    // wire share one phase between them.
+    auto is_phase = [](const LOp &o) {
+        return o.kind == TOP_DIAG && o.k == 1 && o.cmask == 0 && !o.is_dot() && o.coef.size() == 2;
+    };
+    std::vector<char> gone(out.size(), 0);
+    for (size_t i = 0; i < out.size(); ++i) {
+        if (!is_phase(out[i])) continue;
+        const uint64_t bit = 1ull << out[i].bits[0];
+        const size_t lim = std::min(out.size(), i + 4096);
+        for (size_t j = i + 1; j < lim; ++j) {
+            const LOp &o = out[j];
+            if (gone[j]) continue;
+            if (o.is_dot()) break;
+            if (!(o.smask & out[i].smask)) continue;
+            if (is_phase(o) && o.bits[0] == out[i].bits[0] && o.smask == out[i].smask) {
+                out[j].coef[0] *= out[i].coef[0];
+                out[j].coef[1] *= out[i].coef[1];
+                gone[i] = 1;
+                break;
+            }
+            if ((o.nmask() & bit) || o.smask != out[i].smask) break;
+        }
+    }
+    prog.clear();
+    for (size_t i = 0; i < out.size(); ++i)
+        if (!gone[i]) prog.push_back(std::move(out[i]));
+}
+
 bool try_merge_1q(std::vector<LOp> &queue, const LOp &op) {
     if (!single_bit_1q(op)) return false;
     const uint64_t mine = touch_mask(op);
@@ -421,9 +495,163 @@ void to_register_form(TileOp &t, const LOp &o, const TileStage &st, const int *p
     }
 }
 
+bool diagonal_kind(uint32_t k) {
+    return k == TOP_DIAG || k == TOP_SIGN || k == TOP_CZSET || k == TOP_PARITY || k == TOP_PCPHASE ||
+           k == TOP_RPHASE;
+}
+
+// Order of a register stage's ops by earliest level: an op's level is one
+// more than that of the latest earlier op it does not commute with. Ops
+// commute when they act on different states, or when neither acts
+// non-diagonally on a bit the other touches. Dots and ops without a known
+// bit set keep their place relative to everything.
+std::vector<int> stage_levels_order(const std::vector<TileOp> &ops) {
+    const int M = static_cast<int>(ops.size());
+    const uint64_t ALL = ~0ull;
+    std::vector<uint64_t> nd(static_cast<size_t>(M)), dg(static_cast<size_t>(M));
+    for (int x = 0; x < M; ++x) {
+        const TileOp &t = ops[x];
+        uint64_t a = 0, d = t.gcmask;
+        if (t.flags & TF_DOT) {
+            a = ALL;
+        } else if (t.kind == TOP_DENSE1) {
+            a = 1ull << t.gpos[0];
+        } else if (t.kind == TOP_DENSE2) {
+            a = (1ull << t.gpos[0]) | (1ull << t.gpos[1]);
+        } else if (t.kind == TOP_DIAG || t.kind == TOP_PCPHASE) {
+            for (uint32_t j = 0; j < t.k; ++j) d |= 1ull << t.gpos[j];
+        } else if (t.kind == TOP_PARITY) {
+            d |= t.gpmask;
+        } else if (t.kind == TOP_SIGN) {
+        } else {
+            a = ALL;
+        }
+        nd[x] = a;
+        dg[x] = d;
+    }
+    auto commute = [&](int x, int y) {
+        if (nd[x] == ALL || nd[y] == ALL) return false;
+        if (!(ops[x].smask & ops[y].smask)) return true;
+        return !(nd[x] & (nd[y] | dg[y])) && !(nd[y] & (nd[x] | dg[x]));
+    };
+    std::vector<int> level(static_cast<size_t>(M), 0);
+    for (int y = 0; y < M; ++y)
+        for (int x = 0; x < y; ++x)
+            if (!commute(x, y)) level[y] = std::max(level[y], level[x] + 1);
+    std::vector<int> perm(static_cast<size_t>(M));
+    for (int x = 0; x < M; ++x) perm[x] = x;
+    std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return level[x] < level[y]; });
+    return perm;
+}
+
+// In each register stage, one-bit phases on register bits commute with
+// diagonal ops and with dense ops on other register bits, so each can sit
+// anywhere between the ops that block it (a dense op on its bit, a dot, any
+// other non-diagonal op). Choose the fewest points that every phase's range
+// contains (interval stabbing, greedy by right end) and replace the phases at
+// each point by one table over the stage's register index.
+void group_register_phases(PassPlan &p, int rb) {
+    std::vector<TileOp> out;
+    const int E = 1 << rb;
+    for (auto &st : p.stages) {
+        const uint32_t begin = static_cast<uint32_t>(out.size());
+        std::vector<TileOp> ops(p.ops.begin() + st.op_begin, p.ops.begin() + st.op_end);
+        if (st.kind == STAGE_REG) {
+            int ridx[64];
+            for (int b = 0; b < 64; ++b) ridx[b] = -1;
+            for (int i = 0; i < rb; ++i) ridx[p.tbits[st.rbits[i]]] = i;
+            auto is_phase = [&](const TileOp &t) {
+                return t.kind == TOP_DIAG && t.k == 1 && t.gcmask == 0 && !(t.flags & TF_DOT) && t.gpos[0] >= 0 &&
+                       t.gpos[0] < 64 && ridx[t.gpos[0]] >= 0;
+            };
+            auto blocks = [&](const TileOp &l, int r, uint32_t sm) {
+                if (l.flags & TF_DOT) return true;
+                if (diagonal_kind(l.kind)) return false;
+                if (!(l.smask & sm)) return false;
+                if (l.kind == TOP_DENSE1) return l.lpos[0] == r;
+                if (l.kind == TOP_DENSE2) return l.lpos[0] == r || l.lpos[1] == r;
+                return true;
+            };
+            std::vector<TileOp> rest;      // non-phase ops; gap g sits before rest[g]
+            struct Ph { int lo, hi, r; uint32_t sm; cd d0, d1; int at; };
+            std::vector<Ph> ph;
+            for (const TileOp &t : ops) {
+                if (is_phase(t)) {
+                    ph.push_back({0, 0, ridx[t.gpos[0]], t.smask, p.coef[t.coef_off], p.coef[t.coef_off + 1],
+                                  static_cast<int>(rest.size())});
+                } else {
+                    rest.push_back(t);
+                }
+            }
+            const int M = static_cast<int>(rest.size());
+            // each phase's blockers (indices into rest) before and after it
+            std::vector<int> prev_b(ph.size(), -1), next_b(ph.size(), M);
+            for (size_t j = 0; j < ph.size(); ++j) {
+                for (int x = ph[j].at - 1; x >= 0; --x)
+                    if (blocks(rest[x], ph[j].r, ph[j].sm)) { prev_b[j] = x; break; }
+                for (int x = ph[j].at; x < M; ++x)
+                    if (blocks(rest[x], ph[j].r, ph[j].sm)) { next_b[j] = x; break; }
+            }
+            // Reorder the other ops by their earliest level (ops that commute
+            // may pass each other), so that ops on different bits at the same
+            // depth sit together and the phases between them share one point.
+            std::vector<int> perm = stage_levels_order(rest);
+            std::vector<int> newpos(static_cast<size_t>(M));
+            for (int x = 0; x < M; ++x) newpos[perm[x]] = x;
+            {
+                std::vector<TileOp> re(static_cast<size_t>(M));
+                for (int x = 0; x < M; ++x) re[x] = rest[perm[x]];
+                rest.swap(re);
+            }
+            for (size_t j = 0; j < ph.size(); ++j) {
+                ph[j].lo = prev_b[j] < 0 ? 0 : newpos[prev_b[j]] + 1;
+                ph[j].hi = next_b[j] >= M ? M : newpos[next_b[j]];
+            }
+            std::vector<int> order(ph.size());
+            for (size_t i = 0; i < ph.size(); ++i) order[i] = static_cast<int>(i);
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ph[x].hi < ph[y].hi; });
+            std::vector<int> point(ph.size(), -1);
+            for (int i : order) {
+                if (point[i] >= 0) continue;
+                const int g = ph[i].hi;
+                for (size_t j = 0; j < ph.size(); ++j)
+                    if (point[j] < 0 && ph[j].sm == ph[i].sm && ph[j].lo <= g && g <= ph[j].hi) point[j] = g;
+            }
+            std::vector<TileOp> merged;
+            for (int g = 0; g <= M; ++g) {
+                std::vector<uint32_t> sms;
+                for (size_t j = 0; j < ph.size(); ++j)
+                    if (point[j] == g && std::find(sms.begin(), sms.end(), ph[j].sm) == sms.end()) sms.push_back(ph[j].sm);
+                for (uint32_t sm : sms) {
+                    std::vector<cd> table(static_cast<size_t>(E), cd(1.0));
+                    TileOp t{};
+                    for (size_t j = 0; j < ph.size(); ++j) {
+                        if (point[j] != g || ph[j].sm != sm) continue;
+                        for (int e = 0; e < E; ++e) table[e] *= ((e >> ph[j].r) & 1) ? ph[j].d1 : ph[j].d0;
+                    }
+                    t.kind = TOP_RPHASE;
+                    t.k = static_cast<uint32_t>(rb);
+                    t.smask = sm;
+                    for (int j = 0; j < kMaxTileOpTargets; ++j) t.lpos[j] = t.gpos[j] = -1;
+                    t.coef_off = p.coef.size();
+                    p.coef.insert(p.coef.end(), table.begin(), table.end());
+                    merged.push_back(t);
+                }
+                if (g < M) merged.push_back(rest[g]);
+            }
+            ops.swap(merged);
+        }
+        out.insert(out.end(), ops.begin(), ops.end());
+        st.op_begin = begin;
+        st.op_end = static_cast<uint32_t>(out.size());
+    }
+    p.ops.swap(out);
+}
+
 // Merge runs of CZ sign flips into popcount-parity ops and set how each
 // stage's register mapping is reached.
 void finish_reg_pass(PassPlan &p, int kt) {
+    group_register_phases(p, kt - 8);
     std::vector<TileOp> merged;
     for (auto &st : p.stages) {
         const uint32_t begin = static_cast<uint32_t>(merged.size());

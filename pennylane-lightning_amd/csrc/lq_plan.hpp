@@ -58,6 +58,14 @@ LOp lower_generator(int n, GateId g, const uint32_t *wires, int nw, const uint32
 // touches that bit. Returns true if merged.
 bool try_merge_1q(std::vector<LOp> &queue, const LOp &op);
 
+// Write every uncontrolled 2x2 unitary gate U as D(alpha) R(beta) D(delta):
+// a diagonal phase, a rotation with real coefficients, a diagonal phase
+// (the Z-Y-Z Euler form, global phase folded into the last factor). Register
+// passes then apply the rotation with real arithmetic and apply the phases of
+// a stage's register bits together as one table (TOP_RPHASE). Gates whose
+// factors do not reproduce U to 1e-13 are kept whole.
+void split_1q_rotations(std::vector<LOp> &prog);
+
 struct PassPlan {
     std::vector<int> tbits;        // tile bits, ascending
     std::vector<TileOp> ops;

@@ -31,6 +31,7 @@ enum TileOpKind : uint32_t {
     TOP_PAULI = 7,   // Pauli string: (-i)^ny (-1)^popcnt(i & yzmask) a[i ^ xmask]
     TOP_SIGN = 8,    // negate where (index & gcmask) == gcval  (Z, CZ, multi-controlled Z)
     TOP_CZSET = 9,   // negate where sum_i popcnt(index & (index >> sd[i]) & sm[i]) is odd
+    TOP_RPHASE = 10, // register stages: amplitude at register index e times coef[e] (2^RB table)
 };
 
 // Register-resident tile programs (tile_reg_kernel): a pass is a list of
@@ -54,6 +55,7 @@ struct TileStage {
 enum TileOpFlags : uint32_t {
     TF_DOT = 1u,         // read-only: accumulate sum conj(state[sb]) * (G state[sa]) into slot
     TF_ZERO_UNCTRL = 2u, // zero the amplitudes outside the control subspace (generators)
+    TF_REAL = 4u,        // dense op with real coefficients (imaginary parts zero)
 };
 
 struct alignas(16) TileOp {

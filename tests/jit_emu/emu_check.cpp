@@ -2,7 +2,7 @@
 // planner, generate each pass's kernel source, run it under the CPU emulation
 // of emu_rt.hpp / emu_main.cpp (one host thread per GPU thread, real barriers
 // and wave shuffles), and compare the states and dot products with a naive
-// application of the same ops in program order.
+// application of the same ops (before the rotation split) in program order.
 //
 //   emu_check CASE DTYPE N GRID EMU_DIR WORK_DIR
 //   CASE 0: hardware-efficient layers (Rot + CZ ladders) with named-generator dots
@@ -130,7 +130,10 @@ int main(int argc, char **argv) {
         }
     }
     PlanConfig cfg; cfg.n = n; cfg.ns = ns; cfg.kt_max = 12; cfg.low_bits = dtype ? 4 : 5; cfg.reg_rb = 4; cfg.max_dots = 32;
-    auto passes = plan_passes(prog, cfg);
+    // as the engine does for specialised passes: 2x2 gates as phase / real rotation / phase
+    std::vector<LOp> split = prog;
+    if (!getenv("EMU_NO_SPLIT")) split_1q_rotations(split);
+    auto passes = plan_passes(split, cfg);
     // initial states: random normalised
     std::vector<St> S(ns, St(1ull << n));
     for (auto &s : S) for (auto &x : s) x = cd(U(rng) - 3.14, U(rng) - 3.14);
