@@ -389,7 +389,7 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     cfg.ns = ns;
     cfg.kt_max = kt_for(ns);
     cfg.low_bits = t.low_bits >= 0 ? t.low_bits : (dtype_ ? 4 : 5);
-    cfg.budget = t.budget;
+    cfg.budget = t.budget > 0 ? t.budget : 24.0;
     // register-resident passes for tiles of 11 or 12 bits with one or two states
     const int ktr = std::min(cfg.kt_max, 8 + kMaxRegBits);
     if (t.reg && ns <= 2 && n_ >= 11 && ktr >= 11) {
@@ -404,6 +404,7 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     std::vector<LOp> split;
     const bool do_split = cfg.reg_rb > 0 && jcfg.enabled && n_ >= jcfg.min_qubits && jcfg.split_rotations;
     if (do_split) {
+        if (t.budget <= 0) cfg.budget = 36.0;
         split = prog;
         split_1q_rotations(split);
     }
