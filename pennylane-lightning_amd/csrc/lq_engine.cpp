@@ -399,16 +399,46 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     }
     // register passes that will be specialised: 2x2 gates as phase / real
     // rotation / phase, so the rotations take real arithmetic and a stage's
-    // phases apply as one table
+    // phases apply as one table. That form suits only the specialised
+    // kernels (it triples the ops the generic kernel interprets), so it runs
+    // when its kernels are ready; meanwhile their compiles are queued and the
+    // unsplit plan runs in the generic kernel.
     const JitConfig &jcfg = jit_config();
-    std::vector<LOp> split;
-    const bool do_split = cfg.reg_rb > 0 && jcfg.enabled && n_ >= jcfg.min_qubits && jcfg.split_rotations;
-    if (do_split) {
-        if (t.budget <= 0) cfg.budget = 36.0;
-        split = prog;
+    const bool use_jit = cfg.reg_rb > 0 && jcfg.enabled && n_ >= jcfg.min_qubits;
+    std::vector<PassPlan> passes;
+    std::vector<hipFunction_t> fns;
+    auto lookup_all = [&](std::vector<hipFunction_t> &f) {
+        size_t missing = 0;
+        f.assign(passes.size(), nullptr);
+        for (size_t p = 0; p < passes.size(); ++p) {
+            if (!passes[p].reg) continue;
+            f[p] = jit_lookup_plan(passes[p], n_, dtype_, ns);
+            missing += f[p] == nullptr;
+        }
+        return missing;
+    };
+    if (use_jit && jcfg.split_rotations) {
+        PlanConfig scfg = cfg;
+        if (t.budget <= 0) scfg.budget = 36.0;
+        std::vector<LOp> split = prog;
         split_1q_rotations(split);
+        passes = plan_passes(split, scfg);
+        const size_t missing = lookup_all(fns);
+        if (missing) {
+            // a generic pass of the split plan costs about twice one of the
+            // unsplit plan, which costs about 4-5 specialised passes
+            std::vector<PassPlan> plain = plan_passes(prog, cfg);
+            const double split_cost = static_cast<double>(passes.size() - missing) + 9.0 * static_cast<double>(missing);
+            if (4.5 * static_cast<double>(plain.size()) < split_cost) {
+                passes.swap(plain);
+                fns.assign(passes.size(), nullptr);
+            }
+        }
+    } else {
+        passes = plan_passes(prog, cfg);
+        if (use_jit) lookup_all(fns);
+        else fns.assign(passes.size(), nullptr);
     }
-    const std::vector<PassPlan> passes = plan_passes(do_split ? split : prog, cfg);
     if (t.plan_debug) {
         for (size_t p = 0; p < passes.size(); ++p) {
             const PassPlan &pp = passes[p];
@@ -470,15 +500,6 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     hip_check(hipEventRecord(staged_, stream_), "hipEventRecord");
     if (part_n) partials_.ensure(part_n * sizeof(double2));
 
-    // specialised kernels for register passes of large states
-    std::vector<hipFunction_t> fns(passes.size(), nullptr);
-    const JitConfig &jc = jcfg;
-    if (jc.enabled && n_ >= jc.min_qubits) {
-        for (size_t p = 0; p < passes.size(); ++p) {
-            if (!passes[p].reg) continue;
-            fns[p] = jit_lookup_plan(passes[p], n_, dtype_, ns);
-        }
-    }
 
     char *dp = static_cast<char *>(prog_.p);
     for (size_t p = 0; p < passes.size(); ++p) {

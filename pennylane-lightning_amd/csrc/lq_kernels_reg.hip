@@ -13,6 +13,8 @@
 // wave into LDS and written once per tile.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lq_devcommon.hpp"
 #include "lq_kernels.hpp"
 
@@ -44,6 +46,66 @@ __device__ __forceinline__ typename VecT<T>::type pick(const Regs<T, NS, RB> &R,
     return x;
 }
 
+template <int V> using IC = std::integral_constant<int, V>;
+
+// Call f(IC<sa>, IC<sb>) with the dot's ket and bra states as constants (one
+// uniform branch instead of a select per amplitude).
+template <int NS, typename F> __device__ __forceinline__ void with_pair(uint32_t sa, uint32_t sb, F &&f) {
+    if constexpr (NS == 1) {
+        f(IC<0>{}, IC<0>{});
+    } else {
+        if (sa == 0) {
+            if (sb == 1) f(IC<0>{}, IC<1>{});
+            else f(IC<0>{}, IC<0>{});
+        } else {
+            if (sb == 0) f(IC<1>{}, IC<0>{});
+            else f(IC<1>{}, IC<1>{});
+        }
+    }
+}
+
+// One-bit phase diag(d0, d1), no controls, on register bit I of the stage.
+template <typename T, int NS, int RB, int I>
+__device__ __forceinline__ void reg_phase_reg(Regs<T, NS, RB> &R, const TileOp &op, typename VecT<T>::type d0,
+                                              typename VecT<T>::type d1, double &accr, double &acci) {
+    constexpr int E = 1 << RB;
+    if (op.flags & TF_DOT) {
+        with_pair<NS>(op.sa, op.sb, [&](auto A, auto B) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                acc_dot(accr, acci, R.v[decltype(B)::value][e], cmul(((e >> I) & 1) ? d1 : d0, R.v[decltype(A)::value][e]));
+        });
+        return;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (!((op.smask >> s) & 1u)) continue;
+#pragma unroll
+        for (int e = 0; e < E; ++e) R.v[s][e] = cmul(((e >> I) & 1) ? d1 : d0, R.v[s][e]);
+    }
+}
+
+// One-bit phase, no controls, on a bit that is fixed per thread.
+template <typename T, int NS, int RB>
+__device__ __forceinline__ void reg_phase_thread(Regs<T, NS, RB> &R, const TileOp &op, typename VecT<T>::type f,
+                                                 double &accr, double &acci) {
+    constexpr int E = 1 << RB;
+    if (op.flags & TF_DOT) {
+        with_pair<NS>(op.sa, op.sb, [&](auto A, auto B) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                acc_dot(accr, acci, R.v[decltype(B)::value][e], cmul(f, R.v[decltype(A)::value][e]));
+        });
+        return;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (!((op.smask >> s) & 1u)) continue;
+#pragma unroll
+        for (int e = 0; e < E; ++e) R.v[s][e] = cmul(f, R.v[s][e]);
+    }
+}
+
 template <typename T, int NS, int RB, int I>
 __device__ __forceinline__ void reg_dense1(Regs<T, NS, RB> &R, const TileOp &op, const double2 *coef, uint64_t gbt,
                                            const uint64_t (&rg)[kMaxRegBits], double &accr, double &acci) {
@@ -53,15 +115,18 @@ __device__ __forceinline__ void reg_dense1(Regs<T, NS, RB> &R, const TileOp &op,
     const V m10 = ld_coef<T>(coef, op.coef_off + 2), m11 = ld_coef<T>(coef, op.coef_off + 3);
     const uint64_t gcm = op.gcmask, gcv = op.gcval;
     if (op.flags & TF_DOT) {
+        with_pair<NS>(op.sa, op.sb, [&](auto A, auto B) {
+            constexpr int sa = decltype(A)::value, sb = decltype(B)::value;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (e & (1 << I)) continue;
-            const int e1 = e | (1 << I);
-            if ((gidx<RB>(gbt, rg, e) & gcm) != gcv) continue;
-            const V a0 = pick(R, op.sa, e), a1 = pick(R, op.sa, e1);
-            acc_dot(accr, acci, pick(R, op.sb, e), cmul2(m00, a0, m01, a1));
-            acc_dot(accr, acci, pick(R, op.sb, e1), cmul2(m10, a0, m11, a1));
-        }
+            for (int e = 0; e < E; ++e) {
+                if (e & (1 << I)) continue;
+                const int e1 = e | (1 << I);
+                if (gcm != 0 && (gidx<RB>(gbt, rg, e) & gcm) != gcv) continue;
+                const V a0 = R.v[sa][e], a1 = R.v[sa][e1];
+                acc_dot(accr, acci, R.v[sb][e], cmul2(m00, a0, m01, a1));
+                acc_dot(accr, acci, R.v[sb][e1], cmul2(m10, a0, m11, a1));
+            }
+        });
         return;
     }
     if (gcm == 0) {
@@ -182,6 +247,21 @@ __device__ __forceinline__ void reg_diag(Regs<T, NS, RB> &R, const TileOp &op, c
             }
         }
         return;
+    }
+    if (kind == TOP_DIAG && op.k == 1 && gcm == 0) {
+        const V d0 = ld_coef<T>(coef, op.coef_off), d1 = ld_coef<T>(coef, op.coef_off + 1);
+        const uint64_t b = 1ull << op.gpos[0];
+        int r = -1;
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+            if (rg[i] == b) r = i;
+        switch (r) {
+        case 0: reg_phase_reg<T, NS, RB, 0>(R, op, d0, d1, accr, acci); return;
+        case 1: if constexpr (RB > 1) reg_phase_reg<T, NS, RB, 1>(R, op, d0, d1, accr, acci); return;
+        case 2: if constexpr (RB > 2) reg_phase_reg<T, NS, RB, 2>(R, op, d0, d1, accr, acci); return;
+        case 3: if constexpr (RB > 3) reg_phase_reg<T, NS, RB, 3>(R, op, d0, d1, accr, acci); return;
+        default: reg_phase_thread<T, NS, RB>(R, op, (gbt & b) ? d1 : d0, accr, acci); return;
+        }
     }
     const int k = static_cast<int>(op.k);
     V d[4];
