@@ -542,7 +542,9 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
                        (__builtin_popcount(pp.load_mask) + __builtin_popcount(pp.store_mask));
             hip_check(hipEventRecord(pe.a, stream_), "hipEventRecord");
         }
-        if (fns[p]) jit_launch(fns[p], dtype_, ra.sv[0], ra.sv[1], ra.coef, ra.partials, ra.ntiles, lds, stream_);
+        if (fns[p])
+            jit_launch(fns[p], dtype_, ra.sv[0], ra.sv[1], ra.coef, ra.partials, ra.ntiles, jit_lds_bytes(a.g.kt, dtype_, ns),
+                       stream_);
         else if (pp.reg) dispatch(dtype_, [&](auto tt) { launch_tile_reg<decltype(tt)>(ra, lds, stream_); });
         else dispatch(dtype_, [&](auto tt) { launch_tile_pass<decltype(tt)>(a, lds, stream_); });
         ++(fns[p] ? jit_passes_ : generic_passes_);

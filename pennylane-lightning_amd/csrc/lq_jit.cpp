@@ -159,6 +159,10 @@ class Gen {
         E_ = 1 << rb_;
         geom_ = make_geom(n, p.tbits);
         tsize_ = 1u << kt_;
+        const size_t per_state = static_cast<size_t>(dtype ? 16 : 8) << kt_;
+        lds32_ = jit_config().lds32;
+        split_ = lds32_ && per_state > 32768;
+        LQ_ABORT_IF(per_state > 65536, "kernel generator: tile too large");
     }
 
     std::string run() {
@@ -168,7 +172,10 @@ class Gen {
         o << "#define LQ_SWZ " << (dtype_ ? "(l ^ (((l >> 3) ^ (l >> 6) ^ (l >> 9)) & 7u))" : "(l ^ (((l >> 4) ^ (l >> 8)) & 15u))")
           << "\n";
         o << kPrelude;
-        o << "extern \"C\" __global__ __launch_bounds__(256) void lqpass(cplx *__restrict__ sv0, cplx *__restrict__ sv1,"
+        o << "// lds_bytes " << jit_lds_bytes(kt_, dtype_, ns_) << "\n";
+        o << "extern \"C\" __global__ __launch_bounds__(256) ";
+        if (jit_config().waves_per_eu > 0) o << "__attribute__((amdgpu_waves_per_eu(" << jit_config().waves_per_eu << "))) ";
+        o << "void lqpass(cplx *__restrict__ sv0, cplx *__restrict__ sv1,"
              " const double2 *__restrict__ coef, double2 *__restrict__ partials, unsigned long long ntiles) {\n";
         o << "  extern __shared__ cplx lds[];\n";
         const int nslots = static_cast<int>(p_.slot_ids.size());
@@ -299,6 +306,8 @@ class Gen {
     TileGeom geom_;
     Mapping cur_{};
     bool lds_used_ = false;
+    bool lds32_ = false; // states pass through LDS one at a time
+    bool split_ = false; // and each in two halves (complex128 tiles of 2^12)
     std::ostringstream o_;
 
     uint64_t elem_global(const Mapping &m, int e) const {
@@ -330,24 +339,123 @@ class Gen {
     }
     void transpose(const Mapping &to) {
         std::ostringstream &o = o_;
-        if (lds_used_) o << "  __syncthreads();\n";
+        if (!lds32_) {
+            if (lds_used_) o << "  __syncthreads();\n";
+            o << "  {\n";
+            o << "    const unsigned lt = " << deposit("tid", cur_.h, 8, false) << ";\n";
+            for (int s = 0; s < ns_; ++s) {
+                if (!((p_.load_mask >> s) & 1u)) continue;
+                for (int e = 0; e < E_; ++e)
+                    o << "    lds[" << s * tsize_ << "u + swz(lt | " << elem_local(cur_, e) << "u)] = v" << s << "["
+                      << e << "];\n";
+            }
+            o << "  }\n  __syncthreads();\n  {\n";
+            o << "    const unsigned lt = " << deposit("tid", to.h, 8, false) << ";\n";
+            for (int s = 0; s < ns_; ++s) {
+                if (!((p_.load_mask >> s) & 1u)) continue;
+                for (int e = 0; e < E_; ++e)
+                    o << "    v" << s << "[" << e << "] = lds[" << s * tsize_ << "u + swz(lt | " << elem_local(to, e)
+                      << "u)];\n";
+            }
+            o << "  }\n";
+            lds_used_ = true;
+            return;
+        }
+        // 32 KiB of LDS: one state at a time; complex128 tiles in two halves
+        // by a tile bit b. With b a register bit on both sides, every thread
+        // writes and reads half of its amplitudes per round; the values read
+        // go to fresh variables (the old ones of the other half are still to
+        // be written). With b a thread bit at the same thread-index position
+        // on both sides, a thread writes and reads all its amplitudes in one
+        // round, in place. Otherwise the transpose goes through an
+        // intermediate mapping that shares register bits with both sides.
+        if (!split_) {
+            for (int s = 0; s < ns_; ++s) {
+                if (!((p_.load_mask >> s) & 1u)) continue;
+                if (lds_used_) o << "  __syncthreads();\n";
+                lds_used_ = true;
+                half_phase(cur_, s, -1, 0, true, "v");
+                o << "  __syncthreads();\n";
+                half_phase(to, s, -1, 0, false, "v");
+            }
+            return;
+        }
+        int b = -1;
+        for (int i = 0; i < rb_ && b < 0; ++i)
+            for (int j = 0; j < rb_; ++j)
+                if (cur_.r[i] == to.r[j]) b = cur_.r[i];
+        if (b < 0) {
+            for (int j = 0; j < 8; ++j)
+                if (cur_.h[j] == to.h[j]) {
+                    split_transpose(to, cur_.h[j], false);
+                    return;
+                }
+            Mapping mid{};
+            mid.r[0] = cur_.r[0];
+            for (int i = 1; i < rb_; ++i) mid.r[i] = to.r[i - 1];
+            int h = 0;
+            for (int tb = 0; tb < kt_; ++tb) {
+                bool reg = false;
+                for (int i = 0; i < rb_; ++i) reg |= mid.r[i] == tb;
+                if (!reg) mid.h[h++] = tb;
+            }
+            split_transpose(mid, cur_.r[0], true);
+            cur_ = mid;
+            for (int i = 0; i < rb_ && b < 0; ++i)
+                for (int j = 0; j < rb_; ++j)
+                    if (cur_.r[i] == to.r[j]) b = cur_.r[i];
+        }
+        split_transpose(to, b, true);
+    }
+    void split_transpose(const Mapping &to, int b, bool fresh) {
+        std::ostringstream &o = o_;
+        for (int s = 0; s < ns_; ++s) {
+            if (!((p_.load_mask >> s) & 1u)) continue;
+            if (fresh) o << "  {\n  cplx w" << s << "[" << E_ << "];\n";
+            for (int r = 0; r < 2; ++r) {
+                if (lds_used_) o << "  __syncthreads();\n";
+                lds_used_ = true;
+                half_phase(cur_, s, b, r, true, "v");
+                o << "  __syncthreads();\n";
+                half_phase(to, s, b, r, false, fresh ? "w" : "v");
+            }
+            if (fresh) {
+                for (int e = 0; e < E_; ++e) o << "  v" << s << "[" << e << "] = w" << s << "[" << e << "];\n";
+                o << "  }\n";
+            }
+        }
+    }
+    // tile-local index without bit b (b < 0: unchanged)
+    static uint32_t squeeze(uint32_t l, int b) {
+        if (b < 0) return l;
+        return ((l >> (b + 1)) << b) | (l & ((1u << b) - 1u));
+    }
+    // write (or read into `arr`) the amplitudes of state s whose tile bit b
+    // equals r (b < 0: all of them)
+    void half_phase(const Mapping &m, int s, int b, int r, bool write, const char *arr) {
+        std::ostringstream &o = o_;
+        int rb_pos = -1; // register index holding bit b
+        for (int i = 0; i < rb_; ++i)
+            if (m.r[i] == b) rb_pos = i;
         o << "  {\n";
-        o << "    const unsigned lt = " << deposit("tid", cur_.h, 8, false) << ";\n";
-        for (int s = 0; s < ns_; ++s) {
-            if (!((p_.load_mask >> s) & 1u)) continue;
-            for (int e = 0; e < E_; ++e)
-                o << "    lds[" << s * tsize_ << "u + swz(lt | " << elem_local(cur_, e) << "u)] = v" << s << "[" << e
-                  << "];\n";
+        o << "    const unsigned lt = " << deposit("tid", m.h, 8, false) << ";\n";
+        if (b < 0) {
+            o << "    const unsigned lq = lt;\n";
+        } else {
+            o << "    const unsigned lq = ((lt >> " << b + 1 << ") << " << b << ") | (lt & " << ((1u << b) - 1u) << "u);\n";
         }
-        o << "  }\n  __syncthreads();\n  {\n";
-        o << "    const unsigned lt = " << deposit("tid", to.h, 8, false) << ";\n";
-        for (int s = 0; s < ns_; ++s) {
-            if (!((p_.load_mask >> s) & 1u)) continue;
-            for (int e = 0; e < E_; ++e)
-                o << "    v" << s << "[" << e << "] = lds[" << s * tsize_ << "u + swz(lt | " << elem_local(to, e) << "u)];\n";
+        const bool thread_cond = b >= 0 && rb_pos < 0;
+        if (thread_cond) o << "    if (((lt >> " << b << ") & 1u) == " << r << "u) {\n";
+        for (int e = 0; e < E_; ++e) {
+            if (b >= 0 && rb_pos >= 0 && ((e >> rb_pos) & 1) != r) continue;
+            const uint32_t el = squeeze(elem_local(m, e), b);
+            if (write)
+                o << "    lds[swz(lq | " << el << "u)] = v" << s << "[" << e << "];\n";
+            else
+                o << "    " << arr << s << "[" << e << "] = lds[swz(lq | " << el << "u)];\n";
         }
+        if (thread_cond) o << "    }\n";
         o << "  }\n";
-        lds_used_ = true;
     }
 
     // split a (mask, value) condition on global bits into the register part
@@ -640,9 +748,17 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_NT")) j.nontemporal = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_PERSIST")) j.persistent = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_SPLIT")) j.split_rotations = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_JIT_LDS32")) j.lds32 = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_JIT_WAVES")) j.waves_per_eu = std::atoi(s);
         return j;
     }();
     return c;
+}
+
+size_t jit_lds_bytes(int kt, int dtype, int ns) {
+    const size_t per_state = static_cast<size_t>(dtype ? 16 : 8) << kt;
+    if (!jit_config().lds32) return per_state * static_cast<size_t>(ns);
+    return std::min<size_t>(per_state, 32768);
 }
 
 namespace {

@@ -78,7 +78,11 @@ static std::string transform(std::string s) {
         while ((p = s.find(a, p)) != std::string::npos) { s.replace(p, a.size(), b); p += b.size(); ++c; }
         return c;
     };
-    rep("extern __shared__ cplx lds[];", "static cplx lds[16384];");
+    // the kernel states its dynamic LDS bytes; emulate exactly that much
+    const size_t lp = s.find("// lds_bytes ");
+    const unsigned long lds_bytes = lp == std::string::npos ? 0 : std::stoul(s.substr(lp + 13));
+    rep("extern __shared__ cplx lds[];",
+        "static LdsArr<cplx, " + std::to_string(lds_bytes) + "ul / sizeof(cplx)> lds;");
     rep("__shared__ double2 dred[4][32];", "static double2 dred[4][32];");
     rep("asm volatile(\"\" : \"+s\"(z));", "");
     rep("typedef __attribute__((address_space(4))) const double2 kcoef;", "typedef const double2 kcoef;");

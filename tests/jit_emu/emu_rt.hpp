@@ -5,6 +5,8 @@
 #include <barrier>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <thread>
 #include <vector>
 struct double2 { double x, y; };
@@ -22,3 +24,14 @@ extern Dim3 blockIdx, gridDim;
 void __syncthreads();
 double __shfl_xor(double v, int off, int w);
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+// LDS of the size the kernel declares, with every index checked
+template <typename T, unsigned long N> struct LdsArr {
+    T d[N];
+    T &operator[](unsigned long i) {
+        if (i >= N) {
+            std::fprintf(stderr, "LDS index %lu outside %lu entries\n", i, N);
+            std::abort();
+        }
+        return d[i];
+    }
+};
