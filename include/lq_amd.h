@@ -99,6 +99,27 @@ int lq_expval_named_batch(lq_sv_t sv, uint32_t count, const char *const *names, 
 int lq_expval_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, double *out);
 /* expval(Observable) (:372-394) */
 int lq_expval_obs(lq_sv_t sv, lq_obs_t obs, double *out);
+/* var(name, wires) (:461-479): the operation applied by name, |O psi|^2 - Re<psi|O psi>^2 */
+int lq_var_named(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, double *out);
+/* var(matrix, wires) (:488-506); matrix in the state's precision, row-major */
+int lq_var_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, double *out);
+/* var(Observable) (:430-452, calculateObsVar :716-727) */
+int lq_var_obs(lq_sv_t sv, lq_obs_t obs, double *out);
+/* expval / var of a CSR matrix over the state index (:323-337, :616-636;
+ * lq/utils/SparseLinAlg.hpp:42-63): indptr has 2^n + 1 entries, values are
+ * complex in the state's precision */
+int lq_expval_sparse(lq_sv_t sv, const int64_t *indptr, uint64_t indptr_len, const int64_t *indices,
+                     const void *values, uint64_t nnz, double *out);
+int lq_var_sparse(lq_sv_t sv, const int64_t *indptr, uint64_t indptr_len, const int64_t *indices,
+                  const void *values, uint64_t nnz, double *out);
+/* generate_samples(wires, shots) / generate_samples(shots) when all != 0
+ * (:646-677): alias method over probs(wires) (MeasurementKernels.hpp:308-381)
+ * drawn with std::mt19937 seeded with `seed` when seeded != 0 (the
+ * Measurements seed, re-applied on every call as MeasurementsBase.hpp:77-84
+ * does), else from std::random_device. out: shots x nw values 0/1, row-major,
+ * column j = wires[j] */
+int lq_generate_samples(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int all, uint64_t shots, int seeded,
+                        uint64_t seed, uint64_t *out);
 
 /* ---- Observables (core/observables/Observables.hpp:128-584,
  *      lq/observables/ObservablesLQubit.hpp:52-366) ---------------------- */
@@ -107,6 +128,10 @@ int lq_obs_named(const char *name, const uint32_t *wires, uint32_t nw, lq_obs_t 
 int lq_obs_hermitian(const void *matrix_c128, const uint32_t *wires, uint32_t nw, lq_obs_t *out);
 int lq_obs_tensor(const lq_obs_t *factors, uint32_t count, lq_obs_t *out);
 int lq_obs_hamiltonian(const double *coeffs, const lq_obs_t *terms, uint32_t count, lq_obs_t *out);
+/* SparseHamiltonian(data, indices, offsets, wires) (lq/observables/ObservablesLQubit.hpp:372-436):
+ * CSR over the full state index; values complex128 */
+int lq_obs_sparse(const void *values_c128, const int64_t *indices, uint64_t nnz, const int64_t *offsets,
+                  uint64_t offsets_len, const uint32_t *wires, uint32_t nw, lq_obs_t *out);
 int lq_obs_destroy(lq_obs_t obs);
 
 /* ---- Operation list: OpsData (core/algorithms/JacobianData.hpp:39-253),

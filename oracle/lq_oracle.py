@@ -437,6 +437,11 @@ def obs_apply(obs, state: np.ndarray, n: int) -> np.ndarray:
         return state
     if tag == "M":
         return apply_matrix(state, n, obs[1], obs[2])
+    if tag == "S":  # ("S", data, indices, indptr, wires): CSR over the full state index
+        if len(obs[4]) != n:
+            raise ValueError("SparseH wire count does not match state-vector size")
+        state[...] = _sparse_apply(obs[3], obs[2], obs[1], state)
+        return state
     if tag == "H":
         acc = np.zeros_like(state)
         for c, o in zip(obs[1], obs[2]):
@@ -451,6 +456,122 @@ def expval_obs(state: np.ndarray, n: int, obs) -> float:
     (MeasurementsLQubit.hpp:372-394, calculateObsExpval :698-704)."""
     phi = obs_apply(obs, state.copy(), n)
     return float(np.real(np.vdot(state, phi)))
+
+
+def _sparse_apply(indptr, indices, data, state: np.ndarray) -> np.ndarray:
+    """apply_Sparse_Matrix (lq/utils/SparseLinAlg.hpp:42-63): row i of the
+    CSR matrix times the state, rows over the full state index."""
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    data = np.asarray(data, dtype=np.complex128)
+    out = np.zeros(len(indptr) - 1, dtype=np.complex128)
+    rows = np.repeat(np.arange(len(indptr) - 1), np.diff(indptr))
+    np.add.at(out, rows, data * state[indices])
+    return out
+
+
+def var_named(state: np.ndarray, n: int, name: str, wires: Sequence[int]) -> float:
+    """Measurements::var(name, wires) (MeasurementsLQubit.hpp:461-479): the
+    operation applied by name to a copy (Identity skipped),
+    <O psi|O psi> - Re<psi|O psi>^2."""
+    if name == "Identity":
+        phi = state.copy()
+    else:
+        mat = _NAMED_OBS[name] if name in _NAMED_OBS else gate_matrix(name, (), False, len(wires))
+        phi = apply_matrix(state.copy(), n, mat, wires)
+    return float(np.real(np.vdot(phi, phi)) - np.real(np.vdot(state, phi)) ** 2)
+
+
+def var_matrix(state: np.ndarray, n: int, mat: np.ndarray, wires: Sequence[int]) -> float:
+    """Measurements::var(matrix, wires) (MeasurementsLQubit.hpp:488-506)."""
+    phi = apply_matrix(state.copy(), n, np.asarray(mat, dtype=np.complex128), wires)
+    return float(np.real(np.vdot(phi, phi)) - np.real(np.vdot(state, phi)) ** 2)
+
+
+def var_obs(state: np.ndarray, n: int, obs) -> float:
+    """Measurements::var(Observable) (MeasurementsLQubit.hpp:430-452,
+    calculateObsVar :716-727): bra = O psi; <bra|bra> - Re<bra|psi>^2."""
+    bra = obs_apply(obs, state.copy(), n)
+    return float(np.real(np.vdot(bra, bra)) - np.real(np.vdot(bra, state)) ** 2)
+
+
+def expval_sparse(state: np.ndarray, indptr, indices, data) -> float:
+    """Measurements::expval(row_map, col_idx, values) (MeasurementsLQubit.hpp:323-337)."""
+    if len(indptr) - 1 != state.size:
+        raise ValueError("Statevector and Hamiltonian have incompatible sizes.")
+    return float(np.real(np.vdot(state, _sparse_apply(indptr, indices, data, state))))
+
+
+def var_sparse(state: np.ndarray, indptr, indices, data) -> float:
+    """Measurements::var(row_map, col_idx, values) (MeasurementsLQubit.hpp:616-636)."""
+    if len(indptr) - 1 != state.size:
+        raise ValueError("Statevector and Hamiltonian have incompatible sizes.")
+    v = _sparse_apply(indptr, indices, data, state)
+    return float(np.real(np.vdot(v, v)) - np.real(np.vdot(v, state)) ** 2)
+
+
+# Samples. The reference draws with std::mt19937 seeded by the Measurements
+# seed (MeasurementsBase.hpp:77-84) through std::uniform_real_distribution
+# (libstdc++'s generate_canonical: enough 32-bit draws for the mantissa,
+# sum_k g_k 2^(32k) / 2^(32m), clamped below 1) and the alias table of
+# DiscreteRandomVariable (MeasurementKernels.hpp:308-381). numpy's legacy
+# MT19937 seeding is the same init_genrand as std::mt19937(seed).
+class _StdMt19937:
+    def __init__(self, seed: int):
+        self._bg = np.random.MT19937(0)
+        self._bg._legacy_seeding(int(seed) & 0xFFFFFFFF)
+
+    def draw(self) -> int:
+        return int(self._bg.random_raw())
+
+    def canonical(self, single: bool) -> float:
+        if single:  # float: one draw, rounded to float, / 2^32
+            v = np.float32(self.draw()) / np.float32(2.0 ** 32)
+            return float(np.nextafter(np.float32(1), np.float32(0))) if v >= 1 else float(v)
+        s = float(self.draw()) + float(self.draw()) * 2.0 ** 32
+        v = s / 2.0 ** 64
+        return np.nextafter(1.0, 0.0) if v >= 1 else v
+
+
+def _alias_table(p: np.ndarray, single: bool):
+    nprob = len(p)
+    first = [float(np.float32(nprob) * np.float32(x)) if single else float(nprob * float(x)) for x in p]
+    second = [-1] * nprob
+    under = [i for i in range(nprob) if first[i] < 1.0]
+    over = [i for i in range(nprob) if first[i] >= 1.0]
+    while under and over:
+        i = over.pop()
+        j = under.pop()
+        second[j] = i
+        first[i] += first[j] - 1.0
+        (under if first[i] < 1.0 else over).append(i)
+    return first, second
+
+
+def generate_samples(state: np.ndarray, n: int, wires: Sequence[int] | None, shots: int, seed: int,
+                     single: bool = False) -> np.ndarray:
+    """Measurements::generate_samples(wires, shots) with a set seed
+    (MeasurementsLQubit.hpp:646-677): shots x len(wires) of 0/1, column j =
+    wires[j]. ``single`` follows the complex64 engine (float probabilities
+    and draws). Pure-Python loops: small cases only."""
+    w = list(range(n)) if wires is None else list(wires)
+    p = probs(state, n, w)
+    if single:
+        p = p.astype(np.float32)
+    first, second = _alias_table(p, single)
+    rng = _StdMt19937(seed)
+    k = len(w)
+    out = np.zeros((shots, k), dtype=np.uint64)
+    nprob = len(p)
+    for s in range(shots):
+        u = rng.canonical(single)
+        idx = int(np.float32(u) * np.float32(nprob)) if single else int(u * nprob)
+        u2 = rng.canonical(single)
+        if u2 >= first[idx] and second[idx] >= 0:
+            idx = second[idx]
+        for j in range(k):
+            out[s, k - 1 - j] = (idx >> j) & 1
+    return out
 
 
 def obs_to_spec(obs) -> str:

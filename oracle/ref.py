@@ -156,6 +156,47 @@ class RefStateVector:
         _check(lib().ref_expval_obs(self._h(), self.prec, obs_to_spec(obs).encode(), C.byref(out)))
         return out.value
 
+    def var_named(self, name, wires) -> float:
+        w, wp = _u32(wires)
+        out = C.c_double()
+        _check(lib().ref_var_named(self._h(), self.prec, name.encode(), wp, C.c_uint32(len(w)), C.byref(out)))
+        return out.value
+
+    def var_obs(self, obs) -> float:
+        from .lq_oracle import obs_to_spec
+        out = C.c_double()
+        _check(lib().ref_var_obs(self._h(), self.prec, obs_to_spec(obs).encode(), C.byref(out)))
+        return out.value
+
+    def _sparse(self, variance, indptr, indices, data) -> float:
+        ip = np.ascontiguousarray(np.asarray(indptr, dtype=np.int64))
+        ix = np.ascontiguousarray(np.asarray(indices, dtype=np.int64))
+        va = np.ascontiguousarray(np.asarray(data, dtype=np.complex128))
+        out = C.c_double()
+        i64 = C.POINTER(C.c_int64)
+        _check(lib().ref_sparse(self._h(), self.prec, int(variance), ip.ctypes.data_as(i64), C.c_uint64(ip.size),
+                                ix.ctypes.data_as(i64), va.ctypes.data_as(C.c_void_p), C.c_uint64(va.size),
+                                C.byref(out)))
+        return out.value
+
+    def expval_sparse(self, indptr, indices, data) -> float:
+        return self._sparse(False, indptr, indices, data)
+
+    def var_sparse(self, indptr, indices, data) -> float:
+        return self._sparse(True, indptr, indices, data)
+
+    def generate_samples(self, wires, shots: int, seed: int) -> np.ndarray:
+        """shots x len(wires) (all wires when wires is None) after setSeed(seed)."""
+        allw = wires is None
+        wl = [] if allw else list(wires)
+        w, wp = _u32(wl)
+        k = self.n if allw else len(wl)
+        out = np.empty((shots, k), dtype=np.uint64)
+        _check(lib().ref_generate_samples(self._h(), self.prec, wp, C.c_uint32(len(wl)), int(allw),
+                                          C.c_uint64(shots), C.c_uint64(seed),
+                                          out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
     def apply_ops(self, ops):
         for op in ops:
             if op.matrix is not None and op.name not in _known():

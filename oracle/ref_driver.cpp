@@ -10,6 +10,7 @@
 //   applyOperation / applyMatrix  lq/StateVectorLQubit.hpp:378-710
 //   applyGenerator                lq/StateVectorLQubit.hpp:527-560
 //   Measurements<SV>              lq/measurements/MeasurementsLQubit.hpp:72-768
+//                                 (expval, var, sparse expval/var, generate_samples)
 //   NamedObs / TensorProdObs / HermitianObs / Hamiltonian
 //                                 lq/observables/ObservablesLQubit.hpp:52-366
 //   OpsData / JacobianData        core/algorithms/JacobianData.hpp:39-393
@@ -294,6 +295,49 @@ int ref_expval_obs(void *h, int prec, const char *spec, double *out) {
         auto ob = parse_obs<T>(in);
         Measurements<SV<T>> m(sv<T>(h));
         *out = m.expval(*ob);
+    }))
+}
+
+int ref_var_named(void *h, int prec, const char *name, const uint32_t *wires, uint32_t nw, double *out) {
+    GUARD(DISPATCH(prec, {
+        Measurements<SV<T>> m(sv<T>(h));
+        *out = m.var(std::string(name), to_wires<T>(wires, nw));
+    }))
+}
+
+int ref_var_obs(void *h, int prec, const char *spec, double *out) {
+    GUARD(DISPATCH(prec, {
+        std::istringstream in(spec);
+        auto ob = parse_obs<T>(in);
+        Measurements<SV<T>> m(sv<T>(h));
+        *out = m.var(*ob);
+    }))
+}
+
+// CSR operator over the state index; values given as complex128
+int ref_sparse(void *h, int prec, int variance, const int64_t *indptr, uint64_t indptr_len,
+               const int64_t *indices, const void *values_c128, uint64_t nnz, double *out) {
+    GUARD(DISPATCH(prec, {
+        std::vector<std::size_t> rp(indptr, indptr + indptr_len);
+        std::vector<std::size_t> ci(indices, indices + nnz);
+        const auto *v = static_cast<const std::complex<double> *>(values_c128);
+        std::vector<std::complex<T>> vals(nnz);
+        for (uint64_t k = 0; k < nnz; k++) vals[k] = std::complex<T>(static_cast<T>(v[k].real()), static_cast<T>(v[k].imag()));
+        Measurements<SV<T>> m(sv<T>(h));
+        *out = variance ? m.var(rp.data(), rp.size(), ci.data(), vals.data(), vals.size())
+                        : m.expval(rp.data(), rp.size(), ci.data(), vals.data(), vals.size());
+    }))
+}
+
+// generate_samples(wires, shots) (all wires when all != 0) after setSeed(seed)
+int ref_generate_samples(void *h, int prec, const uint32_t *wires, uint32_t nw, int all, uint64_t shots,
+                         uint64_t seed, uint64_t *out) {
+    GUARD(DISPATCH(prec, {
+        Measurements<SV<T>> m(sv<T>(h));
+        m.setSeed(static_cast<std::size_t>(seed));
+        std::vector<std::size_t> s = all ? m.generate_samples(static_cast<std::size_t>(shots))
+                                         : m.generate_samples(to_wires<T>(wires, nw), static_cast<std::size_t>(shots));
+        for (std::size_t i = 0; i < s.size(); i++) out[i] = s[i];
     }))
 }
 

@@ -181,6 +181,60 @@ int lq_expval_obs(lq_sv_t sv, lq_obs_t obs, double *out) {
     });
 }
 
+int lq_var_named(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, double *out) {
+    return guard([&] { *out = S(sv).var_named(name, wires, static_cast<int>(nw)); });
+}
+
+int lq_var_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, double *out) {
+    return guard([&] { *out = S(sv).var_matrix(matrix, wires, static_cast<int>(nw)); });
+}
+
+int lq_var_obs(lq_sv_t sv, lq_obs_t obs, double *out) {
+    return guard([&] {
+        LQ_ABORT_IF(!obs, "null observable");
+        *out = S(sv).var_obs(*obs->obs);
+    });
+}
+
+namespace {
+CsrView csr_args(const int64_t *indptr, uint64_t indptr_len, const int64_t *indices, const void *values, uint64_t nnz) {
+    LQ_ABORT_IF(indptr_len < 1 || !indptr || (nnz && (!indices || !values)), "invalid sparse matrix");
+    return CsrView{indptr, indptr_len - 1, indices, values, nnz};
+}
+} // namespace
+
+int lq_expval_sparse(lq_sv_t sv, const int64_t *indptr, uint64_t indptr_len, const int64_t *indices,
+                     const void *values, uint64_t nnz, double *out) {
+    return guard([&] { *out = S(sv).expval_sparse(csr_args(indptr, indptr_len, indices, values, nnz)); });
+}
+
+int lq_var_sparse(lq_sv_t sv, const int64_t *indptr, uint64_t indptr_len, const int64_t *indices,
+                  const void *values, uint64_t nnz, double *out) {
+    return guard([&] { *out = S(sv).var_sparse(csr_args(indptr, indptr_len, indices, values, nnz)); });
+}
+
+int lq_generate_samples(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int all, uint64_t shots, int seeded,
+                        uint64_t seed, uint64_t *out) {
+    return guard([&] {
+        S(sv).generate_samples(wires, static_cast<int>(nw), all != 0, shots, seeded != 0, seed, out);
+    });
+}
+
+int lq_obs_sparse(const void *values_c128, const int64_t *indices, uint64_t nnz, const int64_t *offsets,
+                  uint64_t offsets_len, const uint32_t *wires, uint32_t nw, lq_obs_t *out) {
+    return guard([&] {
+        LQ_ABORT_IF(offsets_len < 1 || !offsets || (nnz && (!indices || !values_c128)), "invalid sparse matrix");
+        auto o = std::make_shared<Obs>();
+        o->kind = Obs::Sparse;
+        o->wires.assign(wires, wires + nw);
+        const auto *v = static_cast<const cd *>(values_c128);
+        o->sp_values.assign(v, v + nnz);
+        o->sp_indices.assign(indices, indices + nnz);
+        o->sp_indptr.assign(offsets, offsets + offsets_len);
+        *out = new lq_obs_s{o};
+    });
+}
+
 int lq_obs_named(const char *name, const uint32_t *wires, uint32_t nw, lq_obs_t *out) {
     return guard([&] {
         GateId g;

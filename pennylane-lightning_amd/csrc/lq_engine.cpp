@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <random>
 #include <set>
 
 namespace lqamd {
@@ -151,7 +152,7 @@ LOp pauli_dot_op(int n, const PauliWord &w) {
 
 std::vector<uint32_t> Obs::all_wires() const {
     std::set<uint32_t> s;
-    if (kind == Named || kind == Hermitian) s.insert(wires.begin(), wires.end());
+    if (kind == Named || kind == Hermitian || kind == Sparse) s.insert(wires.begin(), wires.end());
     for (const auto &t : terms)
         for (auto w : t->all_wires()) s.insert(w);
     return {s.begin(), s.end()};
@@ -173,7 +174,8 @@ bool obs_to_pauli(const Obs &o, int n, std::vector<PauliWord> *out) {
         } else return false;
         return true;
     }
-    case Obs::Hermitian: return false;
+    case Obs::Hermitian:
+    case Obs::Sparse: return false;
     case Obs::Tensor: {
         std::vector<PauliWord> acc{{1.0, 0, 0, 0}};
         for (const auto &f : o.terms) {
@@ -881,11 +883,26 @@ void StateVector::apply_obs_inplace(const Obs &o, void *state, std::vector<LOp> 
         for (const auto &f : o.terms) apply_obs_inplace(*f, state, prog);
         return;
     case Obs::Ham: LQ_ABORT("a Hamiltonian cannot be applied in place as a product factor");
+    case Obs::Sparse: LQ_ABORT("a sparse Hamiltonian cannot be a product factor");
     }
 }
 
 void StateVector::apply_obs_to(const Obs &o, const void *in, void *out) {
     for (auto w : o.all_wires()) LQ_ABORT_IF(static_cast<int>(w) >= n_, "observable wire out of range");
+    if (o.kind == Obs::Sparse) {
+        LQ_ABORT_IF(static_cast<int>(o.wires.size()) != n_, "SparseH wire count does not match state-vector size");
+        upload_csr(sparse_view(o));
+        const char *base = static_cast<const char *>(sparse_.p);
+        dispatch(dtype_, [&](auto t) {
+            launch_sparse_rows<decltype(t)>(in, out, reinterpret_cast<const int64_t *>(base),
+                                            reinterpret_cast<const int64_t *>(base + csr_indices_off_),
+                                            base + csr_values_off_, length(), 1, nullptr,
+                                            static_cast<uint32_t>(std::min<uint64_t>(4096, std::max<uint64_t>(1, length() / 256))),
+                                            stream_);
+        });
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        return;
+    }
     std::vector<PauliWord> words;
     if (obs_to_pauli(o, n_, &words)) {
         std::vector<PauliTerm> terms;
@@ -911,13 +928,8 @@ void StateVector::apply_obs_to(const Obs &o, const void *in, void *out) {
         // a fresh workspace may hold NaN bit patterns: clear it, do not scale it
         hip_check(hipMemsetAsync(out, 0, bytes(), stream_), "hipMemsetAsync");
         for (size_t i = 0; i < o.terms.size(); ++i) {
-            hip_check(hipMemcpyAsync(tmp, in, bytes(), hipMemcpyDeviceToDevice, stream_), "memcpy");
-            if (o.terms[i]->kind == Obs::Ham) {
-                LQ_ABORT("nested Hamiltonians are not supported");
-            }
-            std::vector<LOp> prog;
-            apply_obs_inplace(*o.terms[i], tmp, prog);
-            run(prog, {tmp}, 0);
+            LQ_ABORT_IF(o.terms[i]->kind == Obs::Ham, "nested Hamiltonians are not supported");
+            apply_obs_to(*o.terms[i], in, tmp);
             dispatch(dtype_, [&](auto t) {
                 launch_axpy<decltype(t)>(tmp, out, o.coeffs[i], 0.0, n_, stream_);
             });
@@ -932,6 +944,12 @@ void StateVector::apply_obs_to(const Obs &o, const void *in, void *out) {
 
 double StateVector::expval_obs(const Obs &o) {
     for (auto w : o.all_wires()) LQ_ABORT_IF(static_cast<int>(w) >= n_, "observable wire out of range");
+    if (o.kind == Obs::Sparse) {
+        LQ_ABORT_IF(static_cast<int>(o.wires.size()) != n_, "SparseH wire count does not match state-vector size");
+        double mean, ms;
+        sparse_moments(sparse_view(o), &mean, &ms);
+        return mean;
+    }
     std::vector<PauliWord> words;
     const bool pauli = obs_to_pauli(o, n_, &words);
     const int ktm = std::min(kt_for(1), n_);
@@ -981,6 +999,228 @@ double StateVector::expval_obs(const Obs &o) {
     void *tmp = workspace(6);
     apply_obs_to(o, d_, tmp);
     return inner(d_, tmp).real();
+}
+
+// ---------------------------------------------------------------------------
+// Variances, sparse Hamiltonians, samples
+// ---------------------------------------------------------------------------
+void StateVector::dense_moments(std::vector<cd> m, const uint32_t *wires, int nw, double *mean, double *mean_square) {
+    LQ_ABORT_IF(nw < 0 || nw > 10, "matrix observables are supported on 1 to 10 wires");
+    check_wires(n_, wires, nw, nullptr, 0);
+    const size_t d = static_cast<size_t>(1) << nw;
+    LQ_ABORT_IF(m.size() != d * d, "The size of matrix does not match with the given number of wires");
+    // <O psi|O psi> = <psi|O^dagger O|psi>
+    std::vector<cd> mm(d * d, 0.0);
+    for (size_t r = 0; r < d; ++r)
+        for (size_t c = 0; c < d; ++c) {
+            cd acc = 0.0;
+            for (size_t k = 0; k < d; ++k) acc += std::conj(m[k * d + r]) * m[k * d + c];
+            mm[r * d + c] = acc;
+        }
+    std::vector<int> tb(static_cast<size_t>(nw));
+    for (int j = 0; j < nw; ++j) tb[j] = phys_bit(n_, wires[j]);
+    // the pending gates and both dots run as one program
+    std::vector<LOp> prog;
+    prog.swap(queue_);
+    prog.push_back(make_dot(lower_matrix(std::move(m), tb, 0, 0, true), 0, 0, 0));
+    prog.push_back(make_dot(lower_matrix(std::move(mm), tb, 0, 0, true), 0, 0, 1));
+    const auto v = run(prog, {d_}, 2);
+    *mean = v[0].real();
+    *mean_square = v[1].real();
+}
+
+double StateVector::var_named(const std::string &name, const uint32_t *wires, int nw) {
+    // the reference applies the operation by name to a copy (Identity is skipped)
+    std::vector<cd> m;
+    if (name == "Identity") {
+        m = {1.0, 0.0, 0.0, 1.0};
+        nw = std::min(nw, 1);
+    } else if (name == "PauliX" || name == "PauliY" || name == "PauliZ" || name == "Hadamard") {
+        LQ_ABORT_IF(nw != 1, "named observables act on one wire");
+        m = named_obs_matrix(name);
+    } else {
+        GateId g;
+        LQ_ABORT_IF(!gate_from_name(name, &g), (std::string("Operation does not exist for ") + name).c_str());
+        LQ_ABORT_IF(gate_info(g).nparams != 0, "parametric operations have no variance without parameters");
+        LQ_ABORT_IF(gate_info(g).nwires > 0 && gate_info(g).nwires != nw, "wrong number of wires for the operation");
+        m = gate_matrix(g, nullptr, false, nw);
+    }
+    double mean, ms;
+    if (nw == 0) {
+        // Identity on no wires: <psi|psi> - <psi|psi>^2
+        const double nn = norm2();
+        return nn - nn * nn;
+    }
+    dense_moments(std::move(m), wires, nw, &mean, &ms);
+    return ms - mean * mean;
+}
+
+double StateVector::var_matrix(const void *mat, const uint32_t *wires, int nw) {
+    LQ_ABORT_IF(nw < 1 || nw > 10, "matrix observables are supported on 1 to 10 wires");
+    const size_t d = static_cast<size_t>(1) << nw;
+    std::vector<cd> m(d * d);
+    if (dtype_) {
+        const auto *src = static_cast<const std::complex<double> *>(mat);
+        for (size_t i = 0; i < d * d; ++i) m[i] = src[i];
+    } else {
+        const auto *src = static_cast<const std::complex<float> *>(mat);
+        for (size_t i = 0; i < d * d; ++i) m[i] = cd(src[i].real(), src[i].imag());
+    }
+    double mean, ms;
+    dense_moments(std::move(m), wires, nw, &mean, &ms);
+    return ms - mean * mean;
+}
+
+double StateVector::var_obs(const Obs &o) {
+    for (auto w : o.all_wires()) LQ_ABORT_IF(static_cast<int>(w) >= n_, "observable wire out of range");
+    if (o.kind == Obs::Sparse) {
+        LQ_ABORT_IF(static_cast<int>(o.wires.size()) != n_, "SparseH wire count does not match state-vector size");
+        const CsrView h = sparse_view(o);
+        double mean, ms;
+        sparse_moments(h, &mean, &ms);
+        return ms - mean * mean;
+    }
+    flush();
+    void *tmp = workspace(6);
+    apply_obs_to(o, d_, tmp);
+    // <O psi|O psi> and Re<O psi|psi> in one read pass
+    const auto v = run({identity_dot(1, 1, 0), identity_dot(0, 1, 1)}, {d_, tmp}, 2);
+    const double mean = v[1].real();
+    return v[0].real() - mean * mean;
+}
+
+CsrView StateVector::sparse_view(const Obs &o) {
+    const size_t nnz = o.sp_values.size();
+    sparse_host_.resize(nnz * elem());
+    if (dtype_) {
+        std::memcpy(sparse_host_.data(), o.sp_values.data(), nnz * sizeof(cd));
+    } else {
+        auto *dst = reinterpret_cast<std::complex<float> *>(sparse_host_.data());
+        for (size_t i = 0; i < nnz; ++i)
+            dst[i] = std::complex<float>(static_cast<float>(o.sp_values[i].real()), static_cast<float>(o.sp_values[i].imag()));
+    }
+    return CsrView{o.sp_indptr.data(), o.sp_indptr.size() - 1, o.sp_indices.data(), sparse_host_.data(), nnz};
+}
+
+void StateVector::sparse_moments(const CsrView &h, double *mean, double *mean_square) {
+    upload_csr(h);
+    const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
+    partials_.ensure(2 * nb * sizeof(double2));
+    const char *base = static_cast<const char *>(sparse_.p);
+    const auto *indptr = reinterpret_cast<const int64_t *>(base);
+    const auto *indices = reinterpret_cast<const int64_t *>(base + csr_indices_off_);
+    const void *values = base + csr_values_off_;
+    dispatch(dtype_, [&](auto t) {
+        launch_sparse_rows<decltype(t)>(d_, nullptr, indptr, indices, values, h.nrows, 0,
+                                        static_cast<double2 *>(partials_.p), nb, stream_);
+    });
+    std::vector<double2> hp(2 * static_cast<size_t>(nb));
+    hip_check(hipMemcpyAsync(hp.data(), partials_.p, hp.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_),
+              "memcpy");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    double s0 = 0.0, c0 = 0.0, s1 = 0.0, c1 = 0.0;
+    auto add = [](double &sum, double &comp, double x) {
+        const double t = sum + x;
+        comp += (std::fabs(sum) >= std::fabs(x)) ? (sum - t) + x : (x - t) + sum;
+        sum = t;
+    };
+    for (uint32_t b = 0; b < nb; ++b) {
+        add(s0, c0, hp[2 * b].x);
+        add(s1, c1, hp[2 * b + 1].x);
+    }
+    *mean = s0 + c0;
+    *mean_square = s1 + c1;
+}
+
+void StateVector::upload_csr(const CsrView &h) {
+    LQ_ABORT_IF(h.nrows != length(), "Statevector and Hamiltonian have incompatible sizes.");
+    // every index the kernel follows is checked here
+    LQ_ABORT_IF(h.indptr[0] != 0, "sparse matrix offsets must start at 0");
+    for (uint64_t i = 0; i < h.nrows; ++i)
+        LQ_ABORT_IF(h.indptr[i + 1] < h.indptr[i], "sparse matrix offsets must not decrease");
+    LQ_ABORT_IF(static_cast<uint64_t>(h.indptr[h.nrows]) != h.nnz, "sparse matrix offsets do not match the number of values");
+    const int64_t len = static_cast<int64_t>(length());
+    for (uint64_t k = 0; k < h.nnz; ++k)
+        LQ_ABORT_IF(h.indices[k] < 0 || h.indices[k] >= len, "sparse matrix column index out of range");
+    flush();
+    auto align = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+    csr_indices_off_ = align((h.nrows + 1) * sizeof(int64_t));
+    csr_values_off_ = align(csr_indices_off_ + h.nnz * sizeof(int64_t));
+    const size_t total = csr_values_off_ + h.nnz * elem() + 16;
+    sparse_.ensure(total);
+    char *base = static_cast<char *>(sparse_.p);
+    hip_check(hipMemcpyAsync(base, h.indptr, (h.nrows + 1) * sizeof(int64_t), hipMemcpyHostToDevice, stream_), "memcpy");
+    if (h.nnz) {
+        hip_check(hipMemcpyAsync(base + csr_indices_off_, h.indices, h.nnz * sizeof(int64_t), hipMemcpyHostToDevice,
+                                 stream_),
+                  "memcpy");
+        hip_check(hipMemcpyAsync(base + csr_values_off_, h.values, h.nnz * elem(), hipMemcpyHostToDevice, stream_),
+                  "memcpy");
+    }
+}
+
+double StateVector::expval_sparse(const CsrView &h) {
+    double mean, ms;
+    sparse_moments(h, &mean, &ms);
+    return mean;
+}
+
+double StateVector::var_sparse(const CsrView &h) {
+    double mean, ms;
+    sparse_moments(h, &mean, &ms);
+    return ms - mean * mean;
+}
+
+namespace {
+// Alias table (Vose's method with LIFO work lists) over probabilities p:
+// entry i keeps i with probability first, else returns its partner
+// (lq/measurements/MeasurementKernels.hpp:347-380).
+template <typename T> std::vector<std::pair<double, size_t>> alias_table(const std::vector<T> &p) {
+    const size_t N = p.size(), none = std::numeric_limits<size_t>::max();
+    std::vector<std::pair<double, size_t>> b(N, {0.0, none});
+    std::vector<size_t> under, over;
+    for (size_t i = 0; i < N; ++i) {
+        b[i].first = static_cast<T>(N) * p[i];
+        (b[i].first < 1.0 ? under : over).push_back(i);
+    }
+    while (!under.empty() && !over.empty()) {
+        const size_t i = over.back();
+        over.pop_back();
+        const size_t j = under.back();
+        under.pop_back();
+        b[j].second = i;
+        b[i].first += b[j].first - 1.0;
+        (b[i].first < 1.0 ? under : over).push_back(i);
+    }
+    return b;
+}
+} // namespace
+
+void StateVector::generate_samples(const uint32_t *wires, int nw, bool all, uint64_t shots, bool seeded, uint64_t seed,
+                                   uint64_t *out) {
+    const std::vector<double> p = probs(wires, nw, all);
+    const int k = all ? n_ : nw;
+    std::mt19937 gen;
+    if (seeded) {
+        gen.seed(static_cast<std::mt19937::result_type>(seed));
+    } else {
+        std::random_device rd;
+        gen.seed(rd());
+    }
+    auto draw = [&](auto tag) {
+        using T = decltype(tag);
+        const std::vector<T> pt(p.begin(), p.end());
+        const auto table = alias_table(pt);
+        const size_t N = pt.size(), none = std::numeric_limits<size_t>::max();
+        std::uniform_real_distribution<T> u(0.0, 1.0);
+        for (uint64_t s = 0; s < shots; ++s) {
+            size_t idx = static_cast<size_t>(u(gen) * static_cast<T>(N));
+            if (u(gen) >= table[idx].first && table[idx].second != none) idx = table[idx].second;
+            for (int j = 0; j < k; ++j) out[s * k + (k - 1 - j)] = (idx >> j) & 1u;
+        }
+    };
+    if (dtype_) draw(double{});
+    else draw(float{});
 }
 
 // ---------------------------------------------------------------------------

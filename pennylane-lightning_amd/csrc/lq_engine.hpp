@@ -40,13 +40,25 @@ struct HostPinned {
 
 // Observables (core/observables/Observables.hpp:128-584)
 struct Obs {
-    enum Kind { Named, Hermitian, Tensor, Ham } kind = Named;
+    enum Kind { Named, Hermitian, Tensor, Ham, Sparse } kind = Named;
     std::string name;
     std::vector<uint32_t> wires;
     std::vector<cd> matrix;                 // Hermitian, row-major complex128
     std::vector<std::shared_ptr<Obs>> terms; // Tensor factors / Hamiltonian terms
     std::vector<double> coeffs;             // Hamiltonian
+    // Sparse: CSR over the full state index (lq/observables/ObservablesLQubit.hpp:372-436)
+    std::vector<cd> sp_values;
+    std::vector<int64_t> sp_indices, sp_indptr;
     std::vector<uint32_t> all_wires() const;
+};
+
+// CSR matrix over the state index (row i = amplitude index i)
+struct CsrView {
+    const int64_t *indptr;   // nrows + 1 entries
+    uint64_t nrows;
+    const int64_t *indices;  // nnz column indices
+    const void *values;      // nnz complex values in the state's precision
+    uint64_t nnz;
 };
 
 struct PauliWord {
@@ -127,6 +139,19 @@ class StateVector {
     double expval_matrix(const void *mat, const uint32_t *wires, int nw);
     double expval_obs(const Obs &o);
     double norm2();
+    // variances (MeasurementsLQubit.hpp:420-536, 604-636): <O psi|O psi> - Re<psi|O psi>^2
+    double var_named(const std::string &name, const uint32_t *wires, int nw);
+    double var_matrix(const void *mat, const uint32_t *wires, int nw);
+    double var_obs(const Obs &o);
+    // sparse Hamiltonian over the state index (MeasurementsLQubit.hpp:323-337, 616-636)
+    double expval_sparse(const CsrView &h);
+    double var_sparse(const CsrView &h);
+    // samples of `wires` (all wires when all) by the alias method over the
+    // marginal probabilities, drawn with std::mt19937 seeded by `seed` when
+    // seeded, else from std::random_device (MeasurementsLQubit.hpp:646-677,
+    // MeasurementKernels.hpp:308-381); out is shots x nw, wires[0] first
+    void generate_samples(const uint32_t *wires, int nw, bool all, uint64_t shots, bool seeded, uint64_t seed,
+                          uint64_t *out);
 
     // queue
     void enqueue(LOp op);
@@ -185,6 +210,16 @@ class StateVector {
     int kt_for(int ns) const;
     std::vector<double> reduce_rows_to_host(uint32_t rows, uint32_t cols);
     void apply_obs_inplace(const Obs &o, void *state, std::vector<LOp> &prog);
+    // Re<psi|O psi> and <O psi|O psi> of a dense k-wire operator, one read pass
+    void dense_moments(std::vector<cd> m, const uint32_t *wires, int nw, double *mean, double *mean_square);
+    // sum conj(psi) H psi and sum |H psi|^2 of a CSR operator, one read pass
+    void sparse_moments(const CsrView &h, double *mean, double *mean_square);
+    // CSR operator on the device: offsets, then column indices, then values
+    DevBuf sparse_;
+    size_t csr_indices_off_ = 0, csr_values_off_ = 0;
+    std::vector<char> sparse_host_;
+    CsrView sparse_view(const Obs &o);
+    void upload_csr(const CsrView &h);
 };
 
 // Lowered op of an OpsList entry: U, or U^dagger with `dagger`

@@ -196,6 +196,45 @@ __global__ __launch_bounds__(kBlock) void overlap_kernel(const typename VecT<TA>
     block_reduce_store(re, im, red, &partials[blockIdx.x]);
 }
 
+// CSR operator rows over the state index (lq/utils/SparseLinAlg.hpp:42-63):
+// v_i = sum_k values[k] a[indices[k]], k in [indptr[i], indptr[i+1]), in FP64.
+// MODE 0: per-block partials of sum conj(a_i) v_i (partials[2b]) and
+// sum |v_i|^2 (partials[2b+1].x); MODE 1: out[i] = v_i.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void sparse_rows_kernel(const typename VecT<T>::type *__restrict__ a,
+                                                             typename VecT<T>::type *__restrict__ out,
+                                                             const int64_t *__restrict__ indptr,
+                                                             const int64_t *__restrict__ indices,
+                                                             const typename VecT<T>::type *__restrict__ values,
+                                                             uint64_t nrows, double2 *partials) {
+    __shared__ double2 red[kBlock / 64];
+    double re = 0.0, im = 0.0, sq = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < nrows; i += stride) {
+        double vr = 0.0, vi = 0.0;
+        const int64_t k1 = indptr[i + 1];
+        for (int64_t k = indptr[i]; k < k1; ++k) {
+            const typename VecT<T>::type h = values[k], x = a[indices[k]];
+            const double hx = h.x, hy = h.y, xx = x.x, xy = x.y;
+            vr += hx * xx - hy * xy;
+            vi += hx * xy + hy * xx;
+        }
+        if (MODE == 1) {
+            out[i] = mk<T>(static_cast<T>(vr), static_cast<T>(vi));
+        } else {
+            const double ax = a[i].x, ay = a[i].y;
+            re += ax * vr + ay * vi;
+            im += ax * vi - ay * vr;
+            sq += vr * vr + vi * vi;
+        }
+    }
+    if (MODE == 0) {
+        block_reduce_store(re, im, red, &partials[2 * static_cast<uint64_t>(blockIdx.x)]);
+        __syncthreads(); // red is reused
+        block_reduce_store(sq, 0.0, red, &partials[2 * static_cast<uint64_t>(blockIdx.x) + 1]);
+    }
+}
+
 struct BitList {
     int32_t b[64];
 };
@@ -398,6 +437,22 @@ void launch_overlap(const void *a, int dtype_a, const void *b, int dtype_b, int 
     else LQ_OV(float, float);
 #undef LQ_OV
 }
+
+template <typename T>
+void launch_sparse_rows(const void *a, void *out, const int64_t *indptr, const int64_t *indices, const void *values,
+                        uint64_t nrows, int mode, double2 *partials, uint32_t nblocks, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    if (mode == 0)
+        hipLaunchKernelGGL((sparse_rows_kernel<T, 0>), dim3(nblocks), dim3(kBlock), 0, s, static_cast<const V *>(a),
+                           nullptr, indptr, indices, static_cast<const V *>(values), nrows, partials);
+    else
+        hipLaunchKernelGGL((sparse_rows_kernel<T, 1>), dim3(nblocks), dim3(kBlock), 0, s, static_cast<const V *>(a),
+                           static_cast<V *>(out), indptr, indices, static_cast<const V *>(values), nrows, nullptr);
+}
+template void launch_sparse_rows<double>(const void *, void *, const int64_t *, const int64_t *, const void *, uint64_t,
+                                         int, double2 *, uint32_t, hipStream_t);
+template void launch_sparse_rows<float>(const void *, void *, const int64_t *, const int64_t *, const void *, uint64_t,
+                                        int, double2 *, uint32_t, hipStream_t);
 
 template <typename T>
 void launch_probs_hist(const void *a, int n, const int *bits, int nw, double *partials, uint32_t nblocks,

@@ -74,6 +74,12 @@ void launch_norm2(const void *a, int n, double2 *partials, uint32_t nblocks, hip
 void launch_overlap(const void *a, int dtype_a, const void *b, int dtype_b, int n, double2 *partials, uint32_t nblocks,
                     hipStream_t s);
 
+// CSR operator over the state index (values in the state's precision):
+// mode 0: partials[2b] = block sums of conj(a_i) (Ha)_i, partials[2b+1].x of
+// |(Ha)_i|^2 (FP64); mode 1: out = H a.
+template <typename T>
+void launch_sparse_rows(const void *a, void *out, const int64_t *indptr, const int64_t *indices, const void *values,
+                        uint64_t nrows, int mode, double2 *partials, uint32_t nblocks, hipStream_t s);
 // marginal probabilities: out[o] over the wires given as physical bits
 // (bits[0] is the output MSB). Small nw: per-block LDS histograms into
 // partials[block][o]; large nw: one output per thread.

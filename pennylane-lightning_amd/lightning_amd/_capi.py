@@ -25,6 +25,7 @@ _U32P = C.POINTER(C.c_uint32)
 _U8P = C.POINTER(C.c_uint8)
 _DP = C.POINTER(C.c_double)
 _U64P = C.POINTER(C.c_uint64)
+_I64P = C.POINTER(C.c_int64)
 
 _SIGS = {
     "lq_last_error": (C.c_char_p, []),
@@ -55,6 +56,13 @@ _SIGS = {
     "lq_expval_named_batch": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_char_p), _U32P, _DP]),
     "lq_expval_matrix": (C.c_int, [_P, _P, _U32P, C.c_uint32, _DP]),
     "lq_expval_obs": (C.c_int, [_P, _P, _DP]),
+    "lq_var_named": (C.c_int, [_P, C.c_char_p, _U32P, C.c_uint32, _DP]),
+    "lq_var_matrix": (C.c_int, [_P, _P, _U32P, C.c_uint32, _DP]),
+    "lq_var_obs": (C.c_int, [_P, _P, _DP]),
+    "lq_expval_sparse": (C.c_int, [_P, _I64P, C.c_uint64, _I64P, _P, C.c_uint64, _DP]),
+    "lq_var_sparse": (C.c_int, [_P, _I64P, C.c_uint64, _I64P, _P, C.c_uint64, _DP]),
+    "lq_generate_samples": (C.c_int, [_P, _U32P, C.c_uint32, C.c_int, C.c_uint64, C.c_int, C.c_uint64, _U64P]),
+    "lq_obs_sparse": (C.c_int, [_P, _I64P, C.c_uint64, _I64P, C.c_uint64, _U32P, C.c_uint32, C.POINTER(_P)]),
     "lq_obs_named": (C.c_int, [C.c_char_p, _U32P, C.c_uint32, C.POINTER(_P)]),
     "lq_obs_hermitian": (C.c_int, [_P, _U32P, C.c_uint32, C.POINTER(_P)]),
     "lq_obs_tensor": (C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(_P)]),

@@ -381,12 +381,102 @@ class Hamiltonian(_Obs):
         return "Hamiltonian: { 'coeffs' : " + repr(self._coeffs) + ", 'observables' : " + repr(self._obs) + "}"
 
 
+def _csr(indptr, indices, values, cdtype):
+    """CSR arrays as the C-ABI takes them (int64 offsets and columns)."""
+    ip = np.ascontiguousarray(np.asarray(indptr, dtype=np.int64).reshape(-1))
+    ix = np.ascontiguousarray(np.asarray(indices, dtype=np.int64).reshape(-1))
+    va = np.ascontiguousarray(np.asarray(values, dtype=cdtype).reshape(-1))
+    if ix.size != va.size:
+        raise LightningException("sparse matrix indices and values differ in length")
+    return ip, ix, va
+
+
+class SparseHamiltonian(_Obs):
+    """SparseHamiltonian(data, indices, offsets, wires): CSR over the full
+    state index (lq/observables/ObservablesLQubit.hpp:372-436)."""
+
+    def __init__(self, data, indices, offsets, wires: Sequence[int]):
+        self._offsets, self._indices, self._data = _csr(offsets, indices, data, np.complex128)
+        self._wires = [int(w) for w in wires]
+        w, wp, nw = u32(self._wires)
+        h = C.c_void_p()
+        i64 = C.POINTER(C.c_int64)
+        check(lib.lq_obs_sparse(self._data.ctypes.data_as(C.c_void_p), self._indices.ctypes.data_as(i64),
+                                self._data.size, self._offsets.ctypes.data_as(i64), self._offsets.size, wp, nw,
+                                C.byref(h)))
+        self._h = h
+
+    def get_wires(self):
+        return list(self._wires)
+
+    def _key(self):
+        return (tuple(self._data.tolist()), tuple(self._indices.tolist()), tuple(self._offsets.tolist()),
+                tuple(self._wires))
+
+    def __repr__(self):
+        return "SparseHamiltonian: {\n'data' : " + repr(self._data.tolist()) + ",\n'indices' : " + \
+            repr(self._indices.tolist()) + ",\n'offsets' : " + repr(self._offsets.tolist()) + "\n}"
+
+
 # ---------------------------------------------------------------------------
 # Measurements (lq/measurements/MeasurementsLQubit.hpp)
 # ---------------------------------------------------------------------------
 class _MeasurementsBase:
     def __init__(self, sv: _StateVectorBase):
         self._sv = sv  # the state vector must outlive the measurements, as in the reference
+        self._seed = None  # MeasurementsBase::_deviceseed (MeasurementsBase.hpp:77-84)
+
+    def set_random_seed(self, seed: int):
+        """Seed for generate_samples; every later call draws from a generator
+        seeded with it again (MeasurementsBase.hpp:77-84)."""
+        self._seed = int(seed)
+
+    def generate_samples(self, wires_or_num_wires, num_shots: int) -> np.ndarray:
+        """generate_samples(num_wires, shots) for all wires (Bindings.hpp:634-640)
+        or generate_samples(wires, shots) (LQubitBindings.hpp:330-336):
+        shots x len(wires) array of 0/1, column j = wires[j]."""
+        shots = int(num_shots)
+        if isinstance(wires_or_num_wires, (int, np.integer)):
+            nw_out = int(wires_or_num_wires)
+            w, wp, nw = u32([])
+            out = np.empty((shots, self._sv._n), dtype=np.uint64)
+            check(lib.lq_generate_samples(self._sv._h, wp, 0, 1, shots, int(self._seed is not None),
+                                          self._seed or 0, out.ctypes.data_as(C.POINTER(C.c_uint64))))
+            return out.reshape(shots, nw_out) if nw_out == self._sv._n else out
+        w, wp, nw = u32(wires_or_num_wires)
+        out = np.empty((shots, nw), dtype=np.uint64)
+        check(lib.lq_generate_samples(self._sv._h, wp, nw, 0, shots, int(self._seed is not None), self._seed or 0,
+                                      out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
+    def _sparse_call(self, fn, indptr, indices, values):
+        ip, ix, va = _csr(indptr, indices, values, self._sv._cdtype)
+        out = C.c_double()
+        i64 = C.POINTER(C.c_int64)
+        check(fn(self._sv._h, ip.ctypes.data_as(i64), ip.size, ix.ctypes.data_as(i64), va.ctypes.data_as(C.c_void_p),
+                 va.size, C.byref(out)))
+        return out.value
+
+    def var(self, *args):
+        """var(observable) | var(name, wires) | var(matrix, wires) |
+        var(row_map, col_idx, values) (MeasurementsLQubit.hpp:420-636)."""
+        out = C.c_double()
+        if len(args) == 1 and isinstance(args[0], _Obs):
+            check(lib.lq_var_obs(self._sv._h, args[0]._h, C.byref(out)))
+        elif len(args) == 2 and isinstance(args[0], str):
+            w, wp, nw = u32(args[1])
+            check(lib.lq_var_named(self._sv._h, args[0].encode(), wp, nw, C.byref(out)))
+        elif len(args) == 2:
+            m = np.ascontiguousarray(np.asarray(args[0], dtype=self._sv._cdtype).reshape(-1))
+            w, wp, nw = u32(args[1])
+            if m.size != (1 << (2 * nw)):
+                raise LightningException("The size of matrix does not match with the given number of wires")
+            check(lib.lq_var_matrix(self._sv._h, m.ctypes.data_as(C.c_void_p), wp, nw, C.byref(out)))
+        elif len(args) == 3:
+            return self._sparse_call(lib.lq_var_sparse, *args)
+        else:
+            raise TypeError("var(observable), var(name, wires), var(matrix, wires) or var(row_map, col_idx, values)")
+        return out.value
 
     def probs(self, wires=None):
         if wires is None:
@@ -412,8 +502,11 @@ class _MeasurementsBase:
             if m.size != (1 << (2 * nw)):
                 raise LightningException("The size of matrix does not match with the given number of wires")
             check(lib.lq_expval_matrix(self._sv._h, m.ctypes.data_as(C.c_void_p), wp, nw, C.byref(out)))
+        elif len(args) == 3:
+            return self._sparse_call(lib.lq_expval_sparse, *args)
         else:
-            raise TypeError("expval(observable), expval(name, wires) or expval(matrix, wires)")
+            raise TypeError("expval(observable), expval(name, wires), expval(matrix, wires) or "
+                            "expval(row_map, col_idx, values)")
         return out.value
 
     def expval_named_batch(self, names: Sequence[str], wires: Sequence[int]) -> np.ndarray:
@@ -557,7 +650,7 @@ algorithms = types.ModuleType(__name__ + ".algorithms")
 sys.modules[observables.__name__] = observables
 sys.modules[algorithms.__name__] = algorithms
 for _bits in (64, 128):
-    for _cls in (NamedObs, HermitianObs, TensorProdObs, Hamiltonian):
+    for _cls in (NamedObs, HermitianObs, TensorProdObs, Hamiltonian, SparseHamiltonian):
         _sub = type(f"{_cls.__name__}C{_bits}", (_cls,), {})
         setattr(observables, _sub.__name__, _sub)
         globals()[_sub.__name__] = _sub
