@@ -77,6 +77,13 @@ int lq_sv_apply_op(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t
 int lq_sv_apply_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
                        const uint8_t *ctrl_vals, uint32_t nc, int inverse);
 /* applyPauliRot(wires, inverse, {theta}, word) (lq/StateVectorLQubit.hpp:492-501) */
+/* applySparseMatrix / applyControlledSparseMatrix (lq/StateVectorLQubit.hpp:725-860,
+ * GateImplementationsLM.hpp:516-574): a CSR 2^nw x 2^nw operator on `wires`
+ * (wires[0] the local MSB), values in the state's precision; inverse is
+ * rejected as in the reference. Applied through the dense fused path (nw <= 10). */
+int lq_sv_apply_sparse_matrix(lq_sv_t sv, const int64_t *indptr, uint64_t indptr_len, const int64_t *indices,
+                              const void *values, uint64_t nnz, const uint32_t *wires, uint32_t nw,
+                              const uint32_t *ctrls, const uint8_t *ctrl_vals, uint32_t nc, int inverse);
 int lq_sv_apply_pauli_rot(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int inverse, double theta,
                           const char *word);
 /* applyGenerator (lq/StateVectorLQubit.hpp:527-560): state <- G state, the

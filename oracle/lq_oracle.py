@@ -470,6 +470,25 @@ def _sparse_apply(indptr, indices, data, state: np.ndarray) -> np.ndarray:
     return out
 
 
+def sparse_to_dense(indptr, indices, data, dim: int) -> np.ndarray:
+    """Row-major dense form of a CSR operator (duplicates summed)."""
+    m = np.zeros((dim, dim), dtype=np.complex128)
+    indptr = np.asarray(indptr, dtype=np.int64)
+    for r in range(dim):
+        for k in range(indptr[r], indptr[r + 1]):
+            m[r, int(indices[k])] += data[k]
+    return m
+
+
+def apply_sparse_matrix(state: np.ndarray, n: int, indptr, indices, data, wires: Sequence[int],
+                        ctrls: Sequence[int] = (), cvals: Sequence[int] = ()) -> np.ndarray:
+    """applyControlledSparseMatrix / applySparseMatrix
+    (GateImplementationsLM.hpp:516-574): the CSR rows over the local index of
+    `wires` (wires[0] the MSB), on the control subspace; no inverse."""
+    m = sparse_to_dense(indptr, indices, data, 1 << len(wires))
+    return apply_matrix(state, n, m, wires, ctrls, cvals)
+
+
 def var_named(state: np.ndarray, n: int, name: str, wires: Sequence[int]) -> float:
     """Measurements::var(name, wires) (MeasurementsLQubit.hpp:461-479): the
     operation applied by name to a copy (Identity skipped),

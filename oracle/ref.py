@@ -156,6 +156,18 @@ class RefStateVector:
         _check(lib().ref_expval_obs(self._h(), self.prec, obs_to_spec(obs).encode(), C.byref(out)))
         return out.value
 
+    def apply_sparse(self, indptr, indices, data, wires, ctrls=(), cvals=()):
+        ip = np.ascontiguousarray(np.asarray(indptr, dtype=np.int64))
+        ix = np.ascontiguousarray(np.asarray(indices, dtype=np.int64))
+        va = np.ascontiguousarray(np.asarray(data, dtype=np.complex128))
+        w, wp = _u32(wires)
+        c, cp = _u32(ctrls)
+        v, vp = _u8(cvals if len(cvals) else [1] * len(ctrls))
+        i64 = C.POINTER(C.c_int64)
+        _check(lib().ref_apply_sparse(self._h(), self.prec, ip.ctypes.data_as(i64), C.c_uint64(ip.size),
+                                      ix.ctypes.data_as(i64), va.ctypes.data_as(C.c_void_p), C.c_uint64(va.size),
+                                      wp, C.c_uint32(len(w)), cp, vp, C.c_uint32(len(c))))
+
     def var_named(self, name, wires) -> float:
         w, wp = _u32(wires)
         out = C.c_double()

@@ -298,6 +298,24 @@ int ref_expval_obs(void *h, int prec, const char *spec, double *out) {
     }))
 }
 
+// applyControlledSparseMatrix (applySparseMatrix without controls); values complex128
+int ref_apply_sparse(void *h, int prec, const int64_t *indptr, uint64_t indptr_len, const int64_t *indices,
+                     const void *values_c128, uint64_t nnz, const uint32_t *wires, uint32_t nw,
+                     const uint32_t *ctrls, const uint8_t *cvals, uint32_t nc) {
+    GUARD(DISPATCH(prec, {
+        std::vector<std::size_t> rp(indptr, indptr + indptr_len);
+        std::vector<std::size_t> ci(indices, indices + nnz);
+        const auto *v = static_cast<const std::complex<double> *>(values_c128);
+        std::vector<std::complex<T>> vals(nnz);
+        for (uint64_t k = 0; k < nnz; k++) vals[k] = std::complex<T>(static_cast<T>(v[k].real()), static_cast<T>(v[k].imag()));
+        if (nc)
+            sv<T>(h).applyControlledSparseMatrix(rp.data(), ci.data(), vals.data(), to_wires<T>(ctrls, nc),
+                                                 to_bools(cvals, nc), to_wires<T>(wires, nw), false);
+        else
+            sv<T>(h).applySparseMatrix(rp.data(), ci.data(), vals.data(), to_wires<T>(wires, nw), false);
+    }))
+}
+
 int ref_var_named(void *h, int prec, const char *name, const uint32_t *wires, uint32_t nw, double *out) {
     GUARD(DISPATCH(prec, {
         Measurements<SV<T>> m(sv<T>(h));

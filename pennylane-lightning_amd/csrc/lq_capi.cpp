@@ -128,6 +128,40 @@ int lq_sv_apply_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, ui
     });
 }
 
+int lq_sv_apply_sparse_matrix(lq_sv_t sv, const int64_t *indptr, uint64_t indptr_len, const int64_t *indices,
+                              const void *values, uint64_t nnz, const uint32_t *wires, uint32_t nw,
+                              const uint32_t *ctrls, const uint8_t *ctrl_vals, uint32_t nc, int inverse) {
+    return guard([&] {
+        LQ_ABORT_IF(nw < 1, "Number of wires must be larger than 0");
+        LQ_ABORT_IF(inverse != 0, "Inverse not implemented for sparse ops.");
+        LQ_ABORT_IF(nw > 10, "sparse matrices are supported on at most 10 wires");
+        const uint64_t d = 1ull << nw;
+        LQ_ABORT_IF(indptr_len != d + 1, "The size of matrix does not match with the given number of wires");
+        LQ_ABORT_IF(indptr[0] != 0, "sparse matrix offsets must start at 0");
+        for (uint64_t i = 0; i < d; ++i)
+            LQ_ABORT_IF(indptr[i + 1] < indptr[i], "sparse matrix offsets must not decrease");
+        LQ_ABORT_IF(static_cast<uint64_t>(indptr[d]) != nnz, "sparse matrix offsets do not match the number of values");
+        // the row-major dense form of the operator, in the state's precision
+        StateVector &st = S(sv);
+        const size_t el = st.elem();
+        std::vector<char> dense(static_cast<size_t>(d * d) * el, 0);
+        for (uint64_t r = 0; r < d; ++r)
+            for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+                LQ_ABORT_IF(indices[k] < 0 || static_cast<uint64_t>(indices[k]) >= d,
+                            "sparse matrix column index out of range");
+                const size_t at = static_cast<size_t>(r * d + static_cast<uint64_t>(indices[k]));
+                if (el == 16) {
+                    auto *m = reinterpret_cast<std::complex<double> *>(dense.data());
+                    m[at] += static_cast<const std::complex<double> *>(values)[k];
+                } else {
+                    auto *m = reinterpret_cast<std::complex<float> *>(dense.data());
+                    m[at] += static_cast<const std::complex<float> *>(values)[k];
+                }
+            }
+        st.apply_matrix(dense.data(), wires, static_cast<int>(nw), ctrls, ctrl_vals, static_cast<int>(nc), false);
+    });
+}
+
 int lq_sv_apply_pauli_rot(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int inverse, double theta,
                           const char *word) {
     return guard([&] { S(sv).apply_pauli_rot(wires, static_cast<int>(nw), inverse != 0, theta, word); });

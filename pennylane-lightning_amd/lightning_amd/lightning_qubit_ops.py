@@ -210,6 +210,25 @@ class _StateVectorBase:
         check(lib.lq_sv_apply_matrix(self._h, m.ctypes.data_as(C.c_void_p), wp, nw, cp, vp, nc,
                                      int(bool(inverse))))
 
+    def applySparseMatrix(self, row_map, col_idx, values, wires, inverse=False):
+        """CSR operator on `wires` (LQubitBindings.hpp:150-161)."""
+        self.applyControlledSparseMatrix(row_map, col_idx, values, [], [], wires, inverse)
+
+    def applyControlledSparseMatrix(self, row_map, col_idx, values, controlled_wires, controlled_values, wires,
+                                    inverse=False):
+        """CSR operator on `wires` where the controls hold their values
+        (LQubitBindings.hpp:163-176)."""
+        ip, ix, va = _csr(row_map, col_idx, values, self._cdtype)
+        w, wp, nw = u32(wires)
+        c, cp, nc = u32(controlled_wires)
+        if len(controlled_values) != nc:
+            raise LightningException("`controlled_wires` must have the same size as `controlled_values`.")
+        v, vp = u8(controlled_values)
+        i64 = C.POINTER(C.c_int64)
+        check(lib.lq_sv_apply_sparse_matrix(self._h, ip.ctypes.data_as(i64), ip.size, ix.ctypes.data_as(i64),
+                                            va.ctypes.data_as(C.c_void_p), va.size, wp, nw, cp, vp, nc,
+                                            int(bool(inverse))))
+
     def applyPauliRot(self, wires, inverse, params, word):
         w, wp, nw = u32(wires)
         check(lib.lq_sv_apply_pauli_rot(self._h, wp, nw, int(bool(inverse)), float(params[0]), word.encode()))
@@ -271,6 +290,7 @@ class StateVectorC64(_StateVectorBase):
 
 class StateVectorC128(_StateVectorBase):
     _bits = 128
+
 
 
 # ---------------------------------------------------------------------------

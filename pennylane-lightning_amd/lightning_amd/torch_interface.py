@@ -130,3 +130,4 @@ def circuit_expvals(params: torch.Tensor, n: int, ops, observables, bits: int = 
     if params.numel() != num_params(ops):
         raise ValueError(f"expected {num_params(ops)} parameters, got {params.numel()}")
     return _CircuitExpvals.apply(params.reshape(-1), n, ops, list(observables), bits)
+

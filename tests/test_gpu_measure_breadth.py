@@ -158,3 +158,26 @@ def test_samples_statistics_at_scale():
     freq = np.bincount(idx, minlength=8) / shots
     p = M.probs([0, 5, 19])
     assert (np.abs(freq - p) <= 5 * np.sqrt(p * (1 - p) / shots) + 1e-12).all()
+
+
+@pytest.mark.parametrize("bits", [128, 64])
+def test_apply_sparse_matrix(bits):
+    n = 12
+    rng = np.random.default_rng(51)
+    for k, ctrls in ((1, []), (3, []), (2, [7]), (5, [0, 11])):
+        sv, psi, _ = state(bits, n, 60 + k)
+        wires = [int(w) for w in rng.permutation(n) if w not in ctrls][:k]
+        cvals = [bool(x) for x in rng.integers(0, 2, len(ctrls))]
+        indptr, cols, vals = random_csr(k, 3 << k, rng)
+        vq = vals.astype(CDT[bits]).astype(np.complex128)
+        if ctrls:
+            sv.applyControlledSparseMatrix(indptr, cols, vals, ctrls, cvals, wires)
+        else:
+            sv.applySparseMatrix(indptr, cols, vals, wires)
+        exp = O.apply_sparse_matrix(psi.copy(), n, indptr, cols, vq, wires, ctrls, [int(c) for c in cvals])
+        got = np.empty(1 << n, dtype=CDT[bits])
+        sv.getState(got)
+        np.testing.assert_allclose(got, exp, atol=TOL[bits] * 10 * float(np.abs(vals).max()) * 4)
+    with pytest.raises(L.LightningException, match="Inverse not implemented"):
+        sv.applySparseMatrix(indptr, cols, vals, wires, inverse=True)
+

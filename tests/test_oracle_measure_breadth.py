@@ -140,3 +140,23 @@ def test_samples_vs_reference_draw_for_draw(prec, wires, seed):
     a = r.generate_samples(wires, 300, seed)
     b = O.generate_samples(s0, n, wires, 300, seed, single=prec == "float")
     assert np.array_equal(a, b)
+
+
+@needs_ref
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_apply_sparse_matrix_vs_reference(prec):
+    n = 6
+    rng = np.random.default_rng(12)
+    for k, ctrls in ((1, []), (2, []), (3, [5]), (4, [0, 2])):
+        st = rand_state(n, rng)
+        wires = [w for w in rng.permutation(n) if w not in ctrls][:k]
+        wires = [int(w) for w in wires]
+        cvals = [int(x) for x in rng.integers(0, 2, len(ctrls))]
+        indptr, cols, vals, _ = random_csr(k, 3 << k, rng, hermitian=False)
+        r = ref.RefStateVector(n, prec)
+        r.set_state(st)
+        r.apply_sparse(indptr, cols, vals, wires, ctrls, cvals)
+        s0 = st.astype(np.complex64).astype(np.complex128) if prec == "float" else st
+        vq = vals.astype(np.complex64).astype(np.complex128) if prec == "float" else vals
+        exp = O.apply_sparse_matrix(s0.copy(), n, indptr, cols, vq, wires, ctrls, cvals)
+        np.testing.assert_allclose(r.get_state(), exp, atol=1e-12 if prec == "double" else 2e-5)
