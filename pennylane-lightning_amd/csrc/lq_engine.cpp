@@ -381,10 +381,15 @@ void StateVector::synchronize() {
 }
 
 std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector<void *> &states, int nslots) {
-    std::vector<cd> result(static_cast<size_t>(nslots), 0.0);
-    if (prog.empty()) return result;
-    const int ns = static_cast<int>(states.size());
+    if (prog.empty()) return std::vector<cd>(static_cast<size_t>(nslots), 0.0);
+    return execute(plan(prog, static_cast<int>(states.size())), states, nslots);
+}
+
+StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
     LQ_ABORT_IF(ns < 1 || ns > kMaxStates, "too many states in one program");
+    Planned out;
+    out.ns = ns;
+    if (prog.empty()) return out;
     const EngineTuning &t = tuning();
     PlanConfig cfg;
     cfg.n = n_;
@@ -455,6 +460,18 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
                          nlds, ntr, pp.end_transpose);
         }
     }
+    out.passes.swap(passes);
+    out.fns.swap(fns);
+    return out;
+}
+
+std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *> &states, int nslots) {
+    std::vector<cd> result(static_cast<size_t>(nslots), 0.0);
+    const int ns = static_cast<int>(states.size());
+    LQ_ABORT_IF(ns != pl.ns, "program planned for another number of states");
+    const std::vector<PassPlan> &passes = pl.passes;
+    const std::vector<hipFunction_t> &fns = pl.fns;
+    if (passes.empty()) return result;
 
     auto align = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
     std::vector<size_t> op_off(passes.size()), coef_off(passes.size()), part_off(passes.size()),
@@ -1327,8 +1344,17 @@ void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std
     hip_check(hipMemcpyAsync(lam0, sv.data(), sv.bytes(), hipMemcpyDeviceToDevice, sv.stream()), "memcpy");
     if (apply_operations) apply_forward(sv, ops, lam0);
     // one H|lambda> per sweep keeps two tiles in LDS; more observables re-run
-    // the sweep from a saved lambda
+    // the sweep from a saved lambda. Every sweep runs the same program: it is
+    // built and planned once, on the host while the forward pass runs.
     const size_t per_sweep = 1;
+    std::vector<LOp> prog;
+    int exp_slots = 0;
+    if (expvals) {
+        for (size_t j = 0; j < per_sweep; ++j) prog.push_back(identity_dot(0, 1 + static_cast<uint32_t>(j), exp_slots++));
+    }
+    const std::vector<double> scales = backward_sweep(sv.n(), ops, tp, per_sweep, exp_slots, prog);
+    const int nslots = exp_slots + static_cast<int>(ntp * per_sweep);
+    const StateVector::Planned planned = sv.plan(prog, 1 + static_cast<int>(per_sweep));
     std::vector<double> out(nobs * ntp, 0.0);
     for (size_t o0 = 0; o0 < nobs; o0 += per_sweep) {
         const size_t m = std::min(per_sweep, nobs - o0);
@@ -1343,14 +1369,7 @@ void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std
             sv.apply_obs_to(*obs[o0 + j], lam, h);
             states.push_back(h);
         }
-        std::vector<LOp> prog;
-        int exp_slots = 0;
-        if (expvals) {
-            for (size_t j = 0; j < m; ++j) prog.push_back(identity_dot(0, 1 + static_cast<uint32_t>(j), exp_slots++));
-        }
-        const std::vector<double> scales = backward_sweep(sv.n(), ops, tp, m, exp_slots, prog);
-        const int nslots = exp_slots + static_cast<int>(ntp * m);
-        const auto v = sv.run(prog, states, nslots);
+        const auto v = sv.execute(planned, states, nslots);
         for (size_t j = 0; j < m; ++j) {
             if (expvals) expvals[o0 + j] = v[j].real();
             for (size_t t = 0; t < ntp; ++t)
@@ -1374,9 +1393,10 @@ void vector_jacobian_product(StateVector &sv, const OpsList &ops, const void *dy
     hip_check(hipMemcpyAsync(lam, sv.data(), sv.bytes(), hipMemcpyDeviceToDevice, sv.stream()), "memcpy");
     hip_check(hipMemcpyAsync(mu, dy_host, sv.bytes(), hipMemcpyHostToDevice, sv.stream()), "memcpy(dy)");
     if (apply_operations) apply_forward(sv, ops, lam);
+    // planned on the host while the forward pass runs
     std::vector<LOp> prog;
     const std::vector<double> scales = backward_sweep(sv.n(), ops, tp, 1, 0, prog);
-    const auto v = sv.run(prog, {lam, mu}, static_cast<int>(ntp));
+    const auto v = sv.execute(sv.plan(prog, 2), {lam, mu}, static_cast<int>(ntp));
     // w_t = i s <G mu | lambda> = i s <mu | G lambda>
     for (size_t t = 0; t < ntp; ++t) out[t] = cd(0.0, scales[t]) * v[t];
 }

@@ -161,6 +161,15 @@ class StateVector {
     // Run an op program over `states` (device pointers of this size/dtype),
     // return the dot slot values (complex) indexed by slot id.
     std::vector<cd> run(const std::vector<LOp> &prog, const std::vector<void *> &states, int nslots);
+    // The same in two steps, so a program can be planned on the host while
+    // earlier work runs on the GPU: plan for `ns` states, then execute.
+    struct Planned {
+        std::vector<PassPlan> passes;
+        std::vector<hipFunction_t> fns; // specialised kernel per pass, or null
+        int ns = 0;
+    };
+    Planned plan(const std::vector<LOp> &prog, int ns);
+    std::vector<cd> execute(const Planned &pl, const std::vector<void *> &states, int nslots);
 
     // out = O in (observable applied, Hamiltonians summed); in/out device states
     void apply_obs_to(const Obs &o, const void *in, void *out);
