@@ -7,9 +7,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <random>
 #include <set>
+#include <mutex>
+#include <unordered_map>
 
 namespace lqamd {
 
@@ -59,6 +62,7 @@ EngineTuning &tuning() {
         if (const char *s = std::getenv("LQ_LDS_BYTES")) e.lds_bytes = static_cast<size_t>(std::atol(s));
         if (const char *s = std::getenv("LQ_REG")) e.reg = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_PLAN_DEBUG")) e.plan_debug = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_PLAN_CACHE")) e.plan_cache = std::atoi(s) != 0;
         return e;
     }();
     return t;
@@ -385,6 +389,46 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
     return execute(plan(prog, static_cast<int>(states.size())), states, nslots);
 }
 
+namespace {
+// Plans by program structure (process-wide, bounded): a program seen before
+// with other coefficients reuses its plan with the coefficients refilled.
+struct PlanCache {
+    std::mutex mu;
+    std::unordered_map<std::string, std::vector<PassPlan>> map;
+    std::deque<std::string> order;
+    static constexpr size_t kMaxEntries = 64;
+    static PlanCache &get() {
+        static PlanCache c;
+        return c;
+    }
+};
+} // namespace
+
+std::vector<PassPlan> plan_cached(const std::vector<LOp> &prog, const PlanConfig &cfg) {
+    if (!tuning().plan_cache) return plan_passes(prog, cfg);
+    std::string key = plan_structure_key(prog, cfg);
+    PlanCache &c = PlanCache::get();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.map.find(key);
+        if (it != c.map.end()) {
+            std::vector<PassPlan> passes = it->second;
+            refresh_plan_coefs(passes, prog);
+            return passes;
+        }
+    }
+    std::vector<PassPlan> passes = plan_passes(prog, cfg);
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.map.emplace(key, passes).second) {
+        c.order.push_back(std::move(key));
+        if (c.order.size() > PlanCache::kMaxEntries) {
+            c.map.erase(c.order.front());
+            c.order.pop_front();
+        }
+    }
+    return passes;
+}
+
 StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
     LQ_ABORT_IF(ns < 1 || ns > kMaxStates, "too many states in one program");
     Planned out;
@@ -429,12 +473,12 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
         if (t.budget <= 0) scfg.budget = 36.0;
         std::vector<LOp> split = prog;
         split_1q_rotations(split);
-        passes = plan_passes(split, scfg);
+        passes = plan_cached(split, scfg);
         const size_t missing = lookup_all(fns);
         if (missing) {
             // a generic pass of the split plan costs about twice one of the
             // unsplit plan, which costs about 4-5 specialised passes
-            std::vector<PassPlan> plain = plan_passes(prog, cfg);
+            std::vector<PassPlan> plain = plan_cached(prog, cfg);
             const double split_cost = static_cast<double>(passes.size() - missing) + 9.0 * static_cast<double>(missing);
             if (4.5 * static_cast<double>(plain.size()) < split_cost) {
                 passes.swap(plain);
@@ -442,7 +486,7 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
             }
         }
     } else {
-        passes = plan_passes(prog, cfg);
+        passes = plan_cached(prog, cfg);
         if (use_jit) lookup_all(fns);
         else fns.assign(passes.size(), nullptr);
     }

@@ -93,6 +93,7 @@ struct EngineTuning {
     size_t lds_bytes = 65536;
     bool reg = true;     // register-resident passes for large tiles
     bool plan_debug = false;
+    bool plan_cache = true; // reuse plans of programs with the same structure
 };
 EngineTuning &tuning();
 
@@ -230,6 +231,11 @@ class StateVector {
     CsrView sparse_view(const Obs &o);
     void upload_csr(const CsrView &h);
 };
+
+// plan_passes through a bounded process-wide cache keyed by program
+// structure (plan_structure_key); a hit refills the cached plan's
+// coefficients from `prog`
+std::vector<PassPlan> plan_cached(const std::vector<LOp> &prog, const PlanConfig &cfg);
 
 // Lowered op of an OpsList entry: U, or U^dagger with `dagger`
 // (core/algorithms/AdjointJacobianBase.hpp:108-127).

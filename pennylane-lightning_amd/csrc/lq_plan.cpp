@@ -573,12 +573,12 @@ void group_register_phases(PassPlan &p, int rb) {
                 return true;
             };
             std::vector<TileOp> rest;      // non-phase ops; gap g sits before rest[g]
-            struct Ph { int lo, hi, r; uint32_t sm; cd d0, d1; int at; };
+            struct Ph { int lo, hi, r; uint32_t sm; cd d0, d1; int at; uint64_t src; };
             std::vector<Ph> ph;
             for (const TileOp &t : ops) {
                 if (is_phase(t)) {
                     ph.push_back({0, 0, ridx[t.gpos[0]], t.smask, p.coef[t.coef_off], p.coef[t.coef_off + 1],
-                                  static_cast<int>(rest.size())});
+                                  static_cast<int>(rest.size()), t.coef_off});
                 } else {
                     rest.push_back(t);
                 }
@@ -625,9 +625,13 @@ void group_register_phases(PassPlan &p, int rb) {
                 for (uint32_t sm : sms) {
                     std::vector<cd> table(static_cast<size_t>(E), cd(1.0));
                     TileOp t{};
+                    PassPlan::PhaseTable rec;
+                    rec.off = p.coef.size();
+                    rec.rb = rb;
                     for (size_t j = 0; j < ph.size(); ++j) {
                         if (point[j] != g || ph[j].sm != sm) continue;
                         for (int e = 0; e < E; ++e) table[e] *= ((e >> ph[j].r) & 1) ? ph[j].d1 : ph[j].d0;
+                        rec.phases.emplace_back(ph[j].src, ph[j].r);
                     }
                     t.kind = TOP_RPHASE;
                     t.k = static_cast<uint32_t>(rb);
@@ -635,6 +639,8 @@ void group_register_phases(PassPlan &p, int rb) {
                     for (int j = 0; j < kMaxTileOpTargets; ++j) t.lpos[j] = t.gpos[j] = -1;
                     t.coef_off = p.coef.size();
                     p.coef.insert(p.coef.end(), table.begin(), table.end());
+                    p.coef_src.insert(p.coef_src.end(), table.size(), PassPlan::CoefSrc{PassPlan::kNoSrc, 0});
+                    p.tables.push_back(std::move(rec));
                     merged.push_back(t);
                 }
                 if (g < M) merged.push_back(rest[g]);
@@ -912,6 +918,8 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
             t.pc_dim = o.pc_dim;
             t.coef_off = p.coef.size();
             p.coef.insert(p.coef.end(), o.coef.begin(), o.coef.end());
+            for (size_t c = 0; c < o.coef.size(); ++c)
+                p.coef_src.push_back({static_cast<uint32_t>(idx), static_cast<uint32_t>(c)});
             if (o.is_dot()) {
                 t.slot = static_cast<uint32_t>(p.slot_ids.size());
                 p.slot_ids.push_back(o.slot);
@@ -929,6 +937,52 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
         passes.push_back(std::move(p));
     }
     return passes;
+}
+
+std::string plan_structure_key(const std::vector<LOp> &ops, const PlanConfig &cfg) {
+    std::string key;
+    auto put = [&key](const void *p, size_t n) { key.append(static_cast<const char *>(p), n); };
+    const double b = cfg.budget;
+    const int32_t hdr[6] = {cfg.n, cfg.ns, cfg.kt_max, cfg.low_bits, cfg.reg_rb, cfg.max_dots};
+    put(hdr, sizeof(hdr));
+    put(&b, sizeof(b));
+    const cd one(1.0), mone(-1.0);
+    for (const LOp &o : ops) {
+        // value tests made while planning (to_register_form)
+        uint32_t vflags = 0;
+        if (o.kind == TOP_DIAG && !o.is_dot()) {
+            if (o.k == 1 && o.coef.size() == 2 && o.coef[0] == one && o.coef[1] == mone) vflags = 1;
+            else if (o.k == 1 && o.coef.size() == 2 && o.coef[0] == mone && o.coef[1] == one) vflags = 2;
+            else if (o.k == 0 && o.coef.size() == 1 && o.coef[0] == mone) vflags = 3;
+        }
+        const uint32_t w[8] = {o.kind, o.flags, static_cast<uint32_t>(o.k), o.smask, o.sa, o.sb,
+                               static_cast<uint32_t>(o.slot), vflags};
+        put(w, sizeof(w));
+        put(o.bits, sizeof(int) * static_cast<size_t>(std::max(0, std::min(o.k, kMaxTileOpTargets))));
+        const uint64_t m[7] = {o.cmask, o.cval, o.xmask, o.yzmask, o.pmask, o.pc_dim,
+                               static_cast<uint64_t>(o.coef.size()) | (static_cast<uint64_t>(o.ny) << 32)};
+        put(m, sizeof(m));
+    }
+    return key;
+}
+
+void refresh_plan_coefs(std::vector<PassPlan> &passes, const std::vector<LOp> &ops) {
+    for (PassPlan &p : passes) {
+        LQ_ABORT_IF(p.coef_src.size() != p.coef.size(), "plan without coefficient sources");
+        for (size_t i = 0; i < p.coef.size(); ++i) {
+            const PassPlan::CoefSrc &s = p.coef_src[i];
+            if (s.op == PassPlan::kNoSrc) continue;
+            p.coef[i] = ops[s.op].coef[s.idx];
+        }
+        for (const PassPlan::PhaseTable &t : p.tables) {
+            const int E = 1 << t.rb;
+            for (int e = 0; e < E; ++e) {
+                cd v(1.0);
+                for (const auto &ph : t.phases) v *= p.coef[ph.first + (((e >> ph.second) & 1) ? 1 : 0)];
+                p.coef[t.off + static_cast<uint64_t>(e)] = v;
+            }
+        }
+    }
 }
 
 } // namespace lqamd

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "lq_common.hpp"
@@ -76,7 +77,23 @@ struct PassPlan {
     bool reg = false;
     std::vector<TileStage> stages;
     uint32_t end_transpose = 0;
+    // where each coefficient comes from, so a plan can be reused for a
+    // program of the same structure with other coefficients: entry i of coef
+    // is ops[coef_src[i].op].coef[coef_src[i].idx], or (op == kNoSrc) an
+    // entry of a phase table, rebuilt from the entries it multiplies
+    static constexpr uint32_t kNoSrc = 0xffffffffu;
+    struct CoefSrc {
+        uint32_t op, idx;
+    };
+    std::vector<CoefSrc> coef_src;
+    struct PhaseTable {
+        uint64_t off;                                 // first entry in coef
+        int rb;                                       // 2^rb entries
+        std::vector<std::pair<uint64_t, int>> phases; // (coef index of d0, register index)
+    };
+    std::vector<PhaseTable> tables;
 };
+
 
 struct PlanConfig {
     int n = 0;
@@ -89,6 +106,14 @@ struct PlanConfig {
 };
 
 std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig &cfg);
+
+// Structural key of a program (everything planning depends on except the
+// coefficient values, plus the value tests planning makes: one-bit and
+// global diagonals equal to +-1 become sign flips).
+std::string plan_structure_key(const std::vector<LOp> &ops, const PlanConfig &cfg);
+// Refill the coefficients of `passes`, planned for a program of the same
+// structure, from `ops`.
+void refresh_plan_coefs(std::vector<PassPlan> &passes, const std::vector<LOp> &ops);
 
 TileGeom make_geom(int n, const std::vector<int> &tbits);
 

@@ -184,11 +184,15 @@ class _StateVectorBase:
 
     # -- gates ----------------------------------------------------------------
     def _apply(self, name, wires, inverse=False, params=(), ctrls=(), cvals=()):
-        w, wp, nw = u32(wires)
-        c, cp, nc = u32(ctrls)
-        v, vp = u8(cvals if len(cvals) else [1] * nc)
-        p, pp, np_ = f64(params)
-        check(lib.lq_sv_apply_op(self._h, name.encode(), wp, nw, cp, vp, nc, int(bool(inverse)), pp, np_))
+        # small ctypes arrays (cheaper per call than numpy conversions)
+        nw, nc, np_ = len(wires), len(ctrls), len(params)
+        wa = (C.c_uint32 * nw)(*wires) if nw else None
+        ca = (C.c_uint32 * nc)(*ctrls) if nc else None
+        va = (C.c_uint8 * nc)(*[1 if bool(x) else 0 for x in cvals]) if nc and len(cvals) else None
+        pa = (C.c_double * np_)(*params) if np_ else None
+        rc = lib.lq_sv_apply_op(self._h, _name_bytes(name), wa, nw, ca, va, nc, int(bool(inverse)), pa, np_)
+        if rc:
+            check(rc)
 
     def applyMatrix(self, matrix, wires, inverse=False):
         m = np.ascontiguousarray(np.asarray(matrix, dtype=self._cdtype).reshape(-1))
@@ -252,6 +256,16 @@ class _StateVectorBase:
 
     def apply_ops(self, ops: "OpsStruct", adjoint: bool = False):
         check(lib.lq_sv_apply_ops(self._h, ops._h, int(bool(adjoint))))
+
+
+_NAME_BYTES = {}
+
+
+def _name_bytes(name: str) -> bytes:
+    b = _NAME_BYTES.get(name)
+    if b is None:
+        b = _NAME_BYTES[name] = name.encode()
+    return b
 
 
 def _make_gate(name):
