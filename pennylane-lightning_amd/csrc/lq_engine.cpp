@@ -413,8 +413,7 @@ std::vector<PassPlan> plan_cached(const std::vector<LOp> &prog, const PlanConfig
         auto it = c.map.find(key);
         if (it != c.map.end()) {
             std::vector<PassPlan> passes = it->second;
-            refresh_plan_coefs(passes, prog);
-            return passes;
+            if (refresh_plan_coefs(passes, prog)) return passes;
         }
     }
     std::vector<PassPlan> passes = plan_passes(prog, cfg);
@@ -471,6 +470,7 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
     if (use_jit && jcfg.split_rotations) {
         PlanConfig scfg = cfg;
         if (t.budget <= 0) scfg.budget = 36.0;
+        scfg.scale_rotations = dtype_ == 1 && jcfg.scale_rotations;
         std::vector<LOp> split = prog;
         split_1q_rotations(split);
         passes = plan_cached(split, scfg);

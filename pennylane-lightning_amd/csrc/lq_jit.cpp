@@ -58,6 +58,12 @@ __device__ __forceinline__ void acc(double &r, double &i, const cplx b, const cp
     r += bx * vx + by * vy;
     i += bx * vy - by * vx;
 }
+// [[1, -t], [t, 1]] on (a, b): two FMAs per component
+__device__ __forceinline__ void d1t(cplx &a, cplx &b, const real t) {
+  const real ax = a.x, ay = a.y;
+  a.x = fma(-t, b.x, ax); a.y = fma(-t, b.y, ay);
+  b.x = fma(t, ax, b.x); b.y = fma(t, ay, b.y);
+}
 // 2x2 with real coefficients: half the multiply-adds of the complex form
 __device__ __forceinline__ void d1r(cplx &a, cplx &b, const real m0, const real m1, const real m2, const real m3) {
     const cplx x = a, y = b;
@@ -498,6 +504,16 @@ class Gen {
         switch (t.kind) {
         case TOP_DENSE1: {
             const int I = t.lpos[0];
+            if ((t.flags & TF_SCALED) && !dot && !pred) {
+                // [[1, -t], [t, 1]]: the rotation's cosine is in a phase table of the pass
+                o << "    const real tt = ldr(coef, cz + " << off + 2 << ");\n";
+                for (int e = 0; e < E_; ++e) {
+                    if ((e >> I) & 1) continue;
+                    const int e1 = e | (1 << I);
+                    for (int s : sts) o << "    d1t(v" << s << "[" << e << "], v" << s << "[" << e1 << "], tt);\n";
+                }
+                break;
+            }
             if ((t.flags & TF_REAL) && !dot && !pred) {
                 o << "    const real r0 = ldr(coef, cz + " << off << "), r1 = ldr(coef, cz + " << off + 1
                   << "), r2 = ldr(coef, cz + " << off + 2 << "), r3 = ldr(coef, cz + " << off + 3 << ");\n";
@@ -749,6 +765,7 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_PERSIST")) j.persistent = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_SPLIT")) j.split_rotations = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_LDS32")) j.lds32 = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_JIT_SCALE")) j.scale_rotations = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_WAVES")) j.waves_per_eu = std::atoi(s);
         return j;
     }();

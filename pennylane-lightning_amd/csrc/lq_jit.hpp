@@ -28,6 +28,7 @@ struct JitConfig {
     bool split_rotations = true; // 2x2 gates as phase / real rotation / phase in specialised passes
     bool persistent = false;  // workgroups loop over tiles, loading the next while computing
                               // (complex128 only; measured slower than one workgroup per tile)
+    bool scale_rotations = true; // complex128 specialised passes: rotations as [[1, -t], [t, 1]]
     bool lds32 = false;       // transposes through 32 KiB of LDS (halves, one state at a time);
                               // measured no faster than 64 KiB at two workgroups per CU
     int waves_per_eu = 0;     // > 0: ask the compiler to fit that many waves per SIMD

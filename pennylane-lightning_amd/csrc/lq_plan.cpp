@@ -745,6 +745,81 @@ TileGeom make_geom(int n, const std::vector<int> &tbits) {
     return g;
 }
 
+namespace {
+constexpr double kMinCos = 1e-100;
+
+// Convert the recorded (or, at plan time, the eligible) real rotations of a
+// pass to [[1, -t], [t, 1]] and multiply the product of their cosines into
+// the pass's scale table. Returns false if a recorded rotation's cosine is
+// now too small for the stored form.
+bool apply_rotation_scaling(PassPlan &p, bool at_plan_time, int rb) {
+    if (at_plan_time) {
+        p.scaled.clear();
+        p.scale_table = -1;
+        if (!p.reg || p.load_mask != 1u || p.store_mask != 1u || !p.slot_ids.empty() || p.stages.empty() ||
+            p.stages.back().kind != STAGE_REG)
+            return true;
+        for (const TileStage &st : p.stages) {
+            if (st.kind != STAGE_REG) continue;
+            for (uint32_t i = st.op_begin; i < st.op_end; ++i) {
+                TileOp &t = p.ops[i];
+                if (t.kind != TOP_DENSE1 || !(t.flags & TF_REAL) || (t.flags & TF_DOT) || t.smask != 1u ||
+                    t.gcmask != 0 || t.lcmask != 0 || t.ocmask != 0)
+                    continue;
+                if (std::fabs(p.coef[t.coef_off].real()) < kMinCos) continue;
+                t.flags |= TF_SCALED;
+                p.scaled.push_back(t.coef_off);
+            }
+        }
+        if (p.scaled.empty()) return true;
+        // the carrier: the last one-state phase table, or a new one at the end
+        for (const TileStage &st : p.stages) {
+            if (st.kind != STAGE_REG) continue;
+            for (uint32_t i = st.op_begin; i < st.op_end; ++i)
+                if (p.ops[i].kind == TOP_RPHASE && p.ops[i].smask == 1u) p.scale_table = static_cast<int64_t>(p.ops[i].coef_off);
+        }
+        if (p.scale_table < 0) {
+            TileOp t{};
+            t.kind = TOP_RPHASE;
+            t.k = static_cast<uint32_t>(rb);
+            t.smask = 1u;
+            for (int j = 0; j < kMaxTileOpTargets; ++j) t.lpos[j] = t.gpos[j] = -1;
+            t.coef_off = p.coef.size();
+            const size_t E = static_cast<size_t>(1) << rb;
+            p.coef.insert(p.coef.end(), E, cd(1.0));
+            p.coef_src.insert(p.coef_src.end(), E, PassPlan::CoefSrc{PassPlan::kNoSrc, 0});
+            PassPlan::PhaseTable rec;
+            rec.off = t.coef_off;
+            rec.rb = rb;
+            p.tables.push_back(rec);
+            p.scale_table = static_cast<int64_t>(t.coef_off);
+            LQ_ABORT_IF(p.stages.back().op_end != p.ops.size(), "last stage does not end the pass");
+            p.ops.push_back(t);
+            p.stages.back().op_end = static_cast<uint32_t>(p.ops.size());
+        }
+    }
+    if (p.scaled.empty()) return true;
+    double prod = 1.0;
+    for (uint64_t off : p.scaled) {
+        const double c = p.coef[off].real(), s = p.coef[off + 2].real();
+        if (std::fabs(c) < kMinCos) return false;
+        const double t = s / c;
+        p.coef[off] = cd(1.0);
+        p.coef[off + 1] = cd(-t);
+        p.coef[off + 2] = cd(t);
+        p.coef[off + 3] = cd(1.0);
+        prod *= c;
+    }
+    // the table's size from its record
+    int trb = 0;
+    for (const auto &tb : p.tables)
+        if (static_cast<int64_t>(tb.off) == p.scale_table) trb = tb.rb;
+    LQ_ABORT_IF(trb <= 0, "scale table not found");
+    for (int e = 0; e < (1 << trb); ++e) p.coef[static_cast<size_t>(p.scale_table) + static_cast<size_t>(e)] *= prod;
+    return true;
+}
+} // namespace
+
 std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig &cfg) {
     const int N = static_cast<int>(ops.size());
     const int n = cfg.n;
@@ -934,6 +1009,7 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
             if (p.reg) p.stages[stage_cursor].op_end = static_cast<uint32_t>(p.ops.size());
         }
         if (p.reg) finish_reg_pass(p, kt);
+        if (p.reg && cfg.scale_rotations) apply_rotation_scaling(p, true, kt - 8);
         passes.push_back(std::move(p));
     }
     return passes;
@@ -943,13 +1019,16 @@ std::string plan_structure_key(const std::vector<LOp> &ops, const PlanConfig &cf
     std::string key;
     auto put = [&key](const void *p, size_t n) { key.append(static_cast<const char *>(p), n); };
     const double b = cfg.budget;
-    const int32_t hdr[6] = {cfg.n, cfg.ns, cfg.kt_max, cfg.low_bits, cfg.reg_rb, cfg.max_dots};
+    const int32_t hdr[7] = {cfg.n, cfg.ns, cfg.kt_max, cfg.low_bits, cfg.reg_rb, cfg.max_dots,
+                            cfg.scale_rotations ? 1 : 0};
     put(hdr, sizeof(hdr));
     put(&b, sizeof(b));
     const cd one(1.0), mone(-1.0);
     for (const LOp &o : ops) {
         // value tests made while planning (to_register_form)
         uint32_t vflags = 0;
+        if (o.kind == TOP_DENSE1 && (o.flags & TF_REAL) && o.coef.size() == 4 && std::fabs(o.coef[0].real()) < kMinCos)
+            vflags = 4;
         if (o.kind == TOP_DIAG && !o.is_dot()) {
             if (o.k == 1 && o.coef.size() == 2 && o.coef[0] == one && o.coef[1] == mone) vflags = 1;
             else if (o.k == 1 && o.coef.size() == 2 && o.coef[0] == mone && o.coef[1] == one) vflags = 2;
@@ -966,7 +1045,7 @@ std::string plan_structure_key(const std::vector<LOp> &ops, const PlanConfig &cf
     return key;
 }
 
-void refresh_plan_coefs(std::vector<PassPlan> &passes, const std::vector<LOp> &ops) {
+bool refresh_plan_coefs(std::vector<PassPlan> &passes, const std::vector<LOp> &ops) {
     for (PassPlan &p : passes) {
         LQ_ABORT_IF(p.coef_src.size() != p.coef.size(), "plan without coefficient sources");
         for (size_t i = 0; i < p.coef.size(); ++i) {
@@ -982,7 +1061,9 @@ void refresh_plan_coefs(std::vector<PassPlan> &passes, const std::vector<LOp> &o
                 p.coef[t.off + static_cast<uint64_t>(e)] = v;
             }
         }
+        if (!apply_rotation_scaling(p, false, 0)) return false;
     }
+    return true;
 }
 
 } // namespace lqamd

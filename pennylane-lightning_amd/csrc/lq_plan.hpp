@@ -92,6 +92,10 @@ struct PassPlan {
         std::vector<std::pair<uint64_t, int>> phases; // (coef index of d0, register index)
     };
     std::vector<PhaseTable> tables;
+    // rotations kept as [[1, -t], [t, 1]] (coef offsets) and the phase table
+    // that carries the product of their cosines (see PlanConfig::scale_rotations)
+    std::vector<uint64_t> scaled;
+    int64_t scale_table = -1;
 };
 
 
@@ -103,6 +107,13 @@ struct PlanConfig {
     double budget = 24;  // max op cost per pass
     int reg_rb = 0;      // > 0: emit register stages for tiles of 8 + reg_rb bits
     int max_dots = 1 << 30; // dot slots per pass
+    // In one-state register passes without dots, store each real rotation
+    // [[c, -s], [s, c]] as [[1, -t], [t, 1]] with t = s/c and multiply the
+    // product of the cosines into one phase table of the pass: two FMAs per
+    // amplitude instead of a multiply and an FMA per component. Rotations with
+    // |c| below 1e-100 keep the full form. For complex128 states (the
+    // unscaled intermediate amplitudes grow by 1/c per rotation).
+    bool scale_rotations = false;
 };
 
 std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig &cfg);
@@ -113,7 +124,9 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
 std::string plan_structure_key(const std::vector<LOp> &ops, const PlanConfig &cfg);
 // Refill the coefficients of `passes`, planned for a program of the same
 // structure, from `ops`.
-void refresh_plan_coefs(std::vector<PassPlan> &passes, const std::vector<LOp> &ops);
+// Returns false when the new coefficients do not fit the plan's value-dependent
+// choices (a scaled rotation whose cosine is now 0): plan afresh then.
+bool refresh_plan_coefs(std::vector<PassPlan> &passes, const std::vector<LOp> &ops);
 
 TileGeom make_geom(int n, const std::vector<int> &tbits);
 

@@ -56,6 +56,7 @@ enum TileOpFlags : uint32_t {
     TF_DOT = 1u,         // read-only: accumulate sum conj(state[sb]) * (G state[sa]) into slot
     TF_ZERO_UNCTRL = 2u, // zero the amplitudes outside the control subspace (generators)
     TF_REAL = 4u,        // dense op with real coefficients (imaginary parts zero)
+    TF_SCALED = 8u,      // real rotation stored as [[1, -t], [t, 1]]; its cosine is applied once per pass
 };
 
 struct alignas(16) TileOp {

@@ -82,6 +82,7 @@ int main() {
                     cfg.budget = split ? 36 : 24;
                     cfg.reg_rb = 4;
                     cfg.max_dots = 32;
+                    cfg.scale_rotations = split != 0;
                     const std::vector<LOp> a = build(n, ns, s, 100 + s, split != 0);
                     const std::vector<LOp> b = build(n, ns, s, 200 + s, split != 0);
                     ++cases;
