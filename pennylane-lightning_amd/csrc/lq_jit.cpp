@@ -192,8 +192,12 @@ class Gen {
         for (int j = 0; j < 8; ++j) m0.h[j] = j;
         for (int s = 0; s < ns_; ++s) o << "  cplx v" << s << "[" << E_ << "];\n";
         if (!persist_) {
-            o << "  const unsigned long long tile = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * gridDim.x;\n";
-            o << "  if (tile >= ntiles) return;\n";
+            // workgroups are dealt round-robin to the 8 XCDs: let XCD x walk
+            // the x-th eighth of the tiles in order, so neighbouring segments
+            // are streamed by one XCD at a time (ntiles is a power of two)
+            o << "  const unsigned long long bid = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * gridDim.x;\n";
+            o << "  if (bid >= ntiles) return;\n";
+            o << "  const unsigned long long tile = ntiles >= 8ull ? (bid & 7ull) * (ntiles >> 3) + (bid >> 3) : bid;\n";
             o << "  {\n";
             emit_base_gl("tile", "gl0", m0);
             emit_loads("v", "gl0", m0);
