@@ -418,7 +418,8 @@ std::vector<PassPlan> plan_cached(const std::vector<LOp> &prog, const PlanConfig
     }
     std::vector<PassPlan> passes = plan_passes(prog, cfg);
     std::lock_guard<std::mutex> lk(c.mu);
-    if (c.map.emplace(key, passes).second) {
+    // a refused refresh replaces the entry: the angles have moved to the new plan's side
+    if (c.map.insert_or_assign(key, passes).second) {
         c.order.push_back(std::move(key));
         if (c.order.size() > PlanCache::kMaxEntries) {
             c.map.erase(c.order.front());
@@ -470,7 +471,9 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
     if (use_jit && jcfg.split_rotations) {
         PlanConfig scfg = cfg;
         if (t.budget <= 0) scfg.budget = 36.0;
-        scfg.scale_rotations = dtype_ == 1 && jcfg.scale_rotations;
+        scfg.scale_rotations = jcfg.scale_rotations;
+        // complex64: keep the unscaled amplitudes (at most 1) well inside float range
+        scfg.max_scale_growth = dtype_ ? 1e300 : 1e20;
         std::vector<LOp> split = prog;
         split_1q_rotations(split);
         passes = plan_cached(split, scfg);

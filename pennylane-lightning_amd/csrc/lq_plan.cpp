@@ -750,11 +750,13 @@ constexpr double kMinCos = 1e-100;
 
 // Convert the recorded (or, at plan time, the eligible) real rotations of a
 // pass to [[1, -t], [t, 1]] and multiply the product of their cosines into
-// the pass's scale table. Returns false if a recorded rotation's cosine is
-// now too small for the stored form.
-bool apply_rotation_scaling(PassPlan &p, bool at_plan_time, int rb) {
+// the pass's scale table. Returns false if the recorded choice no longer
+// holds for the current angles (a cosine too small for the stored form, or
+// the growth bound now crossed in either direction).
+bool apply_rotation_scaling(PassPlan &p, bool at_plan_time, int rb, double max_growth) {
     if (at_plan_time) {
         p.scaled.clear();
+        p.scale_dropped.clear();
         p.scale_table = -1;
         if (!p.reg || p.load_mask != 1u || p.store_mask != 1u || !p.slot_ids.empty() || p.stages.empty() ||
             p.stages.back().kind != STAGE_REG)
@@ -772,6 +774,17 @@ bool apply_rotation_scaling(PassPlan &p, bool at_plan_time, int rb) {
             }
         }
         if (p.scaled.empty()) return true;
+        p.scale_growth_limit = max_growth;
+        double growth = 1.0;
+        for (uint64_t off : p.scaled) growth /= std::fabs(p.coef[off].real());
+        if (!(growth <= max_growth)) {
+            // too much growth for these angles: the full form throughout; a
+            // refresh whose angles would allow the scaled form asks for a new plan
+            for (const TileStage &st : p.stages)
+                for (uint32_t i = st.op_begin; i < st.op_end; ++i) p.ops[i].flags &= ~static_cast<uint32_t>(TF_SCALED);
+            p.scale_dropped.swap(p.scaled);
+            return true;
+        }
         // the carrier: the last one-state phase table, or a new one at the end
         for (const TileStage &st : p.stages) {
             if (st.kind != STAGE_REG) continue;
@@ -798,7 +811,14 @@ bool apply_rotation_scaling(PassPlan &p, bool at_plan_time, int rb) {
             p.stages.back().op_end = static_cast<uint32_t>(p.ops.size());
         }
     }
+    auto growth_of = [&](const std::vector<uint64_t> &offs) {
+        double g = 1.0;
+        for (uint64_t off : offs) g /= std::fabs(p.coef[off].real());
+        return g;
+    };
+    if (!p.scale_dropped.empty() && growth_of(p.scale_dropped) <= p.scale_growth_limit) return false;
     if (p.scaled.empty()) return true;
+    if (!(growth_of(p.scaled) <= p.scale_growth_limit)) return false;
     double prod = 1.0;
     for (uint64_t off : p.scaled) {
         const double c = p.coef[off].real(), s = p.coef[off + 2].real();
@@ -1009,7 +1029,7 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
             if (p.reg) p.stages[stage_cursor].op_end = static_cast<uint32_t>(p.ops.size());
         }
         if (p.reg) finish_reg_pass(p, kt);
-        if (p.reg && cfg.scale_rotations) apply_rotation_scaling(p, true, kt - 8);
+        if (p.reg && cfg.scale_rotations) apply_rotation_scaling(p, true, kt - 8, cfg.max_scale_growth);
         passes.push_back(std::move(p));
     }
     return passes;
@@ -1023,6 +1043,7 @@ std::string plan_structure_key(const std::vector<LOp> &ops, const PlanConfig &cf
                             cfg.scale_rotations ? 1 : 0};
     put(hdr, sizeof(hdr));
     put(&b, sizeof(b));
+    put(&cfg.max_scale_growth, sizeof(cfg.max_scale_growth));
     const cd one(1.0), mone(-1.0);
     for (const LOp &o : ops) {
         // value tests made while planning (to_register_form)
@@ -1061,7 +1082,7 @@ bool refresh_plan_coefs(std::vector<PassPlan> &passes, const std::vector<LOp> &o
                 p.coef[t.off + static_cast<uint64_t>(e)] = v;
             }
         }
-        if (!apply_rotation_scaling(p, false, 0)) return false;
+        if (!apply_rotation_scaling(p, false, 0, p.scale_growth_limit)) return false;
     }
     return true;
 }

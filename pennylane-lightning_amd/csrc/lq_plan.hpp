@@ -96,6 +96,9 @@ struct PassPlan {
     // that carries the product of their cosines (see PlanConfig::scale_rotations)
     std::vector<uint64_t> scaled;
     int64_t scale_table = -1;
+    double scale_growth_limit = 1e300;
+    // eligible rotations left in the full form because of the growth bound
+    std::vector<uint64_t> scale_dropped;
 };
 
 
@@ -111,9 +114,11 @@ struct PlanConfig {
     // [[c, -s], [s, c]] as [[1, -t], [t, 1]] with t = s/c and multiply the
     // product of the cosines into one phase table of the pass: two FMAs per
     // amplitude instead of a multiply and an FMA per component. Rotations with
-    // |c| below 1e-100 keep the full form. For complex128 states (the
-    // unscaled intermediate amplitudes grow by 1/c per rotation).
+    // |c| below 1e-100 keep the full form. The unscaled intermediate
+    // amplitudes grow by 1/c per rotation: a pass whose product of 1/c would
+    // exceed max_scale_growth keeps every rotation in the full form.
     bool scale_rotations = false;
+    double max_scale_growth = 1e300;
 };
 
 std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig &cfg);

@@ -133,7 +133,8 @@ int main(int argc, char **argv) {
             dot(GateId::RY, 5, 0, ns - 1);
         }
     }
-    PlanConfig cfg; cfg.n = n; cfg.ns = ns; cfg.kt_max = 12; cfg.low_bits = getenv("EMU_LOW_BITS") ? atoi(getenv("EMU_LOW_BITS")) : (dtype ? 4 : 5); cfg.reg_rb = 4; cfg.max_dots = 32; cfg.scale_rotations = dtype == 1 && !getenv("EMU_NO_SCALE");
+    PlanConfig cfg; cfg.n = n; cfg.ns = ns; cfg.kt_max = 12; cfg.low_bits = getenv("EMU_LOW_BITS") ? atoi(getenv("EMU_LOW_BITS")) : (dtype ? 4 : 5); cfg.reg_rb = 4; cfg.max_dots = 32; cfg.scale_rotations = !getenv("EMU_NO_SCALE");
+    cfg.max_scale_growth = dtype ? 1e300 : 1e20; // as the engine configures specialised passes
     // as the engine does for specialised passes: 2x2 gates as phase / real rotation / phase
     std::vector<LOp> split = prog;
     if (!getenv("EMU_NO_SPLIT")) split_1q_rotations(split);
