@@ -63,6 +63,7 @@ EngineTuning &tuning() {
         if (const char *s = std::getenv("LQ_REG")) e.reg = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_PLAN_DEBUG")) e.plan_debug = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_PLAN_CACHE")) e.plan_cache = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_PLAN_LOOKAHEAD")) e.lookahead = std::atoi(s) != 0;
         return e;
     }();
     return t;
@@ -470,10 +471,11 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
     };
     if (use_jit && jcfg.split_rotations) {
         PlanConfig scfg = cfg;
-        if (t.budget <= 0) scfg.budget = 36.0;
+        if (t.budget <= 0) scfg.budget = 48.0;
         scfg.scale_rotations = jcfg.scale_rotations;
         // complex64: keep the unscaled amplitudes (at most 1) well inside float range
         scfg.max_scale_growth = dtype_ ? 1e300 : 1e20;
+        scfg.lookahead = t.lookahead;
         std::vector<LOp> split = prog;
         split_1q_rotations(split);
         passes = plan_cached(split, scfg);

@@ -89,11 +89,12 @@ struct OpsList {
 struct EngineTuning {
     int kt_max = 0;      // 0: from the LDS budget
     int low_bits = -1;   // -1: 256-byte segments
-    double budget = 0.0; // 0: 24 per pass, 36 for specialised passes with split rotations
+    double budget = 0.0; // 0: 24 per pass, 48 for specialised passes with split rotations
     size_t lds_bytes = 65536;
     bool reg = true;     // register-resident passes for large tiles
     bool plan_debug = false;
     bool plan_cache = true; // reuse plans of programs with the same structure
+    bool lookahead = true; // lookahead tile-bit choice for specialised passes
 };
 EngineTuning &tuning();
 

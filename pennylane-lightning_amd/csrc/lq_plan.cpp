@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <queue>
 #include <set>
 
 namespace lqamd {
@@ -898,13 +899,85 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
 
     const int ktmax = std::min(cfg.kt_max, n);
     const int low = std::max(0, std::min(cfg.low_bits, ktmax));
+    const uint64_t low_mask = (low >= 64) ? ~0ull : ((1ull << low) - 1);
+
+    // Lookahead choice of a pass's tile bits. run_with(T) replays, without
+    // committing, the schedule the pass would make with tile bits T (ready ops
+    // in program order, within the cost budget and dot slots) and returns the
+    // cost it fits; ops held back only by bits outside T are collected as the
+    // frontier. Starting from the low bits, the missing bits of the frontier
+    // op whose addition lets the most work run per added bit are added until
+    // the tile is full or the budget is met. On a chain of two-qubit gates
+    // this grows contiguous blocks of wires, which advance several layers per
+    // pass, where adding the first ready op's bits picks every other wire and
+    // advances two.
+    std::vector<int> touched;
+    auto run_with = [&](uint64_t T, std::vector<int> *frontier, bool *full) {
+        std::priority_queue<int, std::vector<int>, std::greater<int>> q(ready.begin(), ready.end());
+        double cost = 0.0;
+        int dots = 0, done = 0;
+        *full = false;
+        while (!q.empty()) {
+            const int i = q.top();
+            q.pop();
+            if ((nm[i] & ~T) != 0 || (dots >= cfg.max_dots && ops[i].is_dot())) {
+                if (frontier) frontier->push_back(i);
+                continue;
+            }
+            if (done > 0 && cost + ops[i].cost() > cfg.budget) {
+                *full = true;
+                break;
+            }
+            cost += ops[i].cost();
+            dots += ops[i].is_dot() ? 1 : 0;
+            ++done;
+            for (int s2 : succ[i]) {
+                touched.push_back(s2);
+                if (--indeg[s2] == 0) q.push(s2);
+            }
+        }
+        for (int s2 : touched) ++indeg[s2];
+        touched.clear();
+        return cost;
+    };
+    auto choose_bits = [&]() {
+        uint64_t T = low_mask;
+        std::vector<int> frontier;
+        while (popc(T) < ktmax) {
+            frontier.clear();
+            bool full = false;
+            const double base = run_with(T, &frontier, &full);
+            if (full) break;
+            uint64_t best = 0;
+            double best_score = 0.0;
+            int best_bits = 0;
+            std::vector<uint64_t> seen;
+            for (int i : frontier) {
+                const uint64_t m = nm[i] & ~T;
+                if (m == 0 || popc(T | m) > ktmax || std::find(seen.begin(), seen.end(), m) != seen.end()) continue;
+                seen.push_back(m);
+                bool f2 = false;
+                const double score = (run_with(T | m, nullptr, &f2) - base) / popc(m);
+                if (score > best_score + 1e-12 || (best != 0 && std::fabs(score - best_score) <= 1e-12 && popc(m) < best_bits)) {
+                    best = m;
+                    best_score = score;
+                    best_bits = popc(m);
+                }
+            }
+            if (best == 0) break;
+            T |= best;
+        }
+        return T;
+    };
+
     std::vector<PassPlan> passes;
     int remaining = N;
     while (remaining > 0) {
         std::vector<int> sel;
         uint64_t T = 0;
+        const uint64_t T0 = cfg.lookahead ? choose_bits() : low_mask;
         for (int attempt = 0; attempt < 2 && sel.empty(); ++attempt) {
-            T = attempt == 0 ? ((low >= 64) ? ~0ull : ((1ull << low) - 1)) : 0ull;
+            T = attempt == 0 ? T0 : 0ull;
             double cost = 0.0;
             int dots = 0;
             while (true) {
@@ -1039,8 +1112,8 @@ std::string plan_structure_key(const std::vector<LOp> &ops, const PlanConfig &cf
     std::string key;
     auto put = [&key](const void *p, size_t n) { key.append(static_cast<const char *>(p), n); };
     const double b = cfg.budget;
-    const int32_t hdr[7] = {cfg.n, cfg.ns, cfg.kt_max, cfg.low_bits, cfg.reg_rb, cfg.max_dots,
-                            cfg.scale_rotations ? 1 : 0};
+    const int32_t hdr[8] = {cfg.n, cfg.ns, cfg.kt_max, cfg.low_bits, cfg.reg_rb, cfg.max_dots,
+                            cfg.scale_rotations ? 1 : 0, cfg.lookahead ? 1 : 0};
     put(hdr, sizeof(hdr));
     put(&b, sizeof(b));
     put(&cfg.max_scale_growth, sizeof(cfg.max_scale_growth));

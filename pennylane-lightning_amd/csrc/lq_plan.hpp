@@ -119,6 +119,9 @@ struct PlanConfig {
     // exceed max_scale_growth keeps every rotation in the full form.
     bool scale_rotations = false;
     double max_scale_growth = 1e300;
+    // choose each pass's tile bits by lookahead (the bits that let the most
+    // work run) instead of taking the first ready op's bits
+    bool lookahead = false;
 };
 
 std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig &cfg);

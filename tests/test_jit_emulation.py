@@ -33,12 +33,14 @@ def emu_binary(tmp_path_factory):
     return exe, str(d)
 
 
-@pytest.mark.parametrize("case,dtype,persistent,lds32", [(0, 1, 0, 0), (1, 1, 0, 0), (2, 1, 0, 0), (1, 0, 0, 0), (2, 0, 0, 0),
-                                                        (2, 1, 1, 0), (0, 1, 0, 1), (2, 1, 0, 1), (2, 0, 0, 1)])
-def test_generated_kernels_match_naive_application(emu_binary, case, dtype, persistent, lds32):
+@pytest.mark.parametrize("case,dtype,persistent,lds32,look", [
+    (0, 1, 0, 0, 1), (1, 1, 0, 0, 1), (2, 1, 0, 0, 1), (1, 0, 0, 0, 1), (2, 0, 0, 0, 1), (2, 1, 1, 0, 1),
+    (0, 1, 0, 1, 1), (2, 1, 0, 1, 1), (2, 0, 0, 1, 1), (0, 1, 0, 0, 0), (1, 0, 0, 0, 0), (2, 1, 0, 0, 0)])
+def test_generated_kernels_match_naive_application(emu_binary, case, dtype, persistent, lds32, look):
     exe, work = emu_binary
-    # lds32: transposes through 32 KiB of LDS (emulated at exactly the size the kernel declares)
-    env = dict(os.environ, LQ_JIT_PERSIST=str(persistent), LQ_JIT_LDS32=str(lds32))
+    # lds32: transposes through 32 KiB of LDS (emulated at exactly the size the kernel declares);
+    # look 0: tile bits from the first ready op instead of the planner's lookahead
+    env = dict(os.environ, LQ_JIT_PERSIST=str(persistent), LQ_JIT_LDS32=str(lds32), EMU_LOOKAHEAD=str(look))
     # 14 qubits: 4 tiles per pass, walked by 3 workgroups in the persistent form
     r = subprocess.run([exe, str(case), str(dtype), "14", "3", EMU, work], capture_output=True, text=True,
                        timeout=600, env=env)
