@@ -180,7 +180,7 @@ class Gen {
         o << kPrelude;
         o << "// lds_bytes " << jit_lds_bytes(kt_, dtype_, ns_) << "\n";
         o << "extern \"C\" __global__ __launch_bounds__(256) ";
-        if (jit_config().waves_per_eu > 0) o << "__attribute__((amdgpu_waves_per_eu(" << jit_config().waves_per_eu << "))) ";
+        if (const int w = jit_waves_per_eu(dtype_, ns_); w > 0) o << "__attribute__((amdgpu_waves_per_eu(" << w << "))) ";
         o << "void lqpass(cplx *__restrict__ sv0, cplx *__restrict__ sv1,"
              " const double2 *__restrict__ coef, double2 *__restrict__ partials, unsigned long long ntiles) {\n";
         o << "  extern __shared__ cplx lds[];\n";
@@ -774,6 +774,16 @@ JitConfig &jit_config() {
         return j;
     }();
     return c;
+}
+
+int jit_waves_per_eu(int dtype, int ns) {
+    const int w = jit_config().waves_per_eu;
+    if (w >= 0) return w;
+    // measured on the 30-qubit HEA passes with 32 KiB transposes: 3 waves per
+    // SIMD for complex128 (2: 2% slower, 4: 10% slower), 4 for complex64;
+    // two-state passes hold twice the registers and get no request
+    if (ns != 1) return 0;
+    return dtype ? 3 : 4;
 }
 
 size_t jit_lds_bytes(int kt, int dtype, int ns) {

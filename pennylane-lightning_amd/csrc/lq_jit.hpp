@@ -28,12 +28,15 @@ struct JitConfig {
     bool split_rotations = true; // 2x2 gates as phase / real rotation / phase in specialised passes
     bool persistent = false;  // workgroups loop over tiles, loading the next while computing
                               // (complex128 only; measured slower than one workgroup per tile)
-    bool scale_rotations = true; // complex128 specialised passes: rotations as [[1, -t], [t, 1]]
-    bool lds32 = false;       // transposes through 32 KiB of LDS (halves, one state at a time);
-                              // measured no faster than 64 KiB at two workgroups per CU
-    int waves_per_eu = 0;     // > 0: ask the compiler to fit that many waves per SIMD
+    bool scale_rotations = true; // specialised passes: rotations as [[1, -t], [t, 1]]
+    bool lds32 = true;        // transposes through 32 KiB of LDS (halves, one state at a time)
+                              // so that more workgroups fit per CU
+    int waves_per_eu = -1;    // > 0: ask the compiler to fit that many waves per SIMD; 0: no
+                              // request; -1: by kind (one-state passes: 3 complex128, 4 complex64)
 };
 JitConfig &jit_config();
+// Waves per SIMD requested of a specialised pass kernel (0: none).
+int jit_waves_per_eu(int dtype, int ns);
 // Dynamic LDS bytes of a specialised register pass kernel.
 size_t jit_lds_bytes(int kt, int dtype, int ns);
 
