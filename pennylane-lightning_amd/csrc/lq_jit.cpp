@@ -586,6 +586,19 @@ class Gen {
             break;
         }
         case TOP_CZSET: emit_czset(t, sts); break;
+        case TOP_TPHASE: {
+            // product of one-bit factors on bits fixed per thread, then one multiply per amplitude
+            for (uint32_t j = 0; j < t.k; ++j) {
+                const std::string pick = "(((gt >> " + std::to_string(t.gpos[j]) + ") & 1ull) ? ldc(coef, cz + " +
+                                         std::to_string(off + 2 * j + 1) + ") : ldc(coef, cz + " +
+                                         std::to_string(off + 2 * j) + "))";
+                if (j == 0) o << "    cplx f = " << pick << ";\n";
+                else o << "    f = cm(" << pick << ", f);\n";
+            }
+            for (int e = 0; e < E_; ++e)
+                for (int s : sts) o << "    v" << s << "[" << e << "] = cm(f, v" << s << "[" << e << "]);\n";
+            break;
+        }
         case TOP_RPHASE: {
             for (int e = 0; e < E_; ++e)
                 for (int s : sts)
@@ -733,7 +746,8 @@ std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns) {
         case TOP_PARITY:
         case TOP_SIGN:
         case TOP_CZSET:
-        case TOP_RPHASE: break;
+        case TOP_RPHASE:
+        case TOP_TPHASE: break;
         case TOP_DIAG:
             if (t.k > 2) return "";
             break;

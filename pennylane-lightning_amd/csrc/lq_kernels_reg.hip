@@ -236,6 +236,19 @@ __device__ __forceinline__ void reg_diag(Regs<T, NS, RB> &R, const TileOp &op, c
         }
         return;
     }
+    if (kind == TOP_TPHASE) {
+        // product of one-bit factors on bits fixed per thread
+        V f = ld_coef<T>(coef, op.coef_off + ((gbt >> op.gpos[0]) & 1ull));
+        for (uint32_t j = 1; j < op.k; ++j)
+            f = cmul(ld_coef<T>(coef, op.coef_off + 2 * j + ((gbt >> op.gpos[j]) & 1ull)), f);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (!((op.smask >> s) & 1u)) continue;
+#pragma unroll
+            for (int e = 0; e < E; ++e) R.v[s][e] = cmul(f, R.v[s][e]);
+        }
+        return;
+    }
     if (kind == TOP_RPHASE) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {

@@ -498,7 +498,7 @@ void to_register_form(TileOp &t, const LOp &o, const TileStage &st, const int *p
 
 bool diagonal_kind(uint32_t k) {
     return k == TOP_DIAG || k == TOP_SIGN || k == TOP_CZSET || k == TOP_PARITY || k == TOP_PCPHASE ||
-           k == TOP_RPHASE;
+           k == TOP_RPHASE || k == TOP_TPHASE;
 }
 
 // Order of a register stage's ops by earliest level: an op's level is one
@@ -519,7 +519,7 @@ std::vector<int> stage_levels_order(const std::vector<TileOp> &ops) {
             a = 1ull << t.gpos[0];
         } else if (t.kind == TOP_DENSE2) {
             a = (1ull << t.gpos[0]) | (1ull << t.gpos[1]);
-        } else if (t.kind == TOP_DIAG || t.kind == TOP_PCPHASE) {
+        } else if (t.kind == TOP_DIAG || t.kind == TOP_PCPHASE || t.kind == TOP_TPHASE) {
             for (uint32_t j = 0; j < t.k; ++j) d |= 1ull << t.gpos[j];
         } else if (t.kind == TOP_PARITY) {
             d |= t.gpmask;
@@ -545,6 +545,65 @@ std::vector<int> stage_levels_order(const std::vector<TileOp> &ops) {
     return perm;
 }
 
+// In a register stage the non-diagonal ops act on register bits only, so a
+// one-bit phase on any other bit (a thread or outside bit: fixed per thread)
+// commutes with every op of the stage but a dot or an op of unknown action.
+// Between two such blockers, the phases of one state set become one
+// TOP_TPHASE op at the first one's place: each thread forms the product of
+// their factors once and multiplies its amplitudes once, instead of once per
+// phase. Coefficients are copied (pairs d0, d1 per phase) with their sources.
+void group_thread_phases(PassPlan &p, std::vector<TileOp> &ops, const int *ridx) {
+    auto is_tphase = [&](const TileOp &t) {
+        return t.kind == TOP_DIAG && t.k == 1 && t.gcmask == 0 && !(t.flags & TF_DOT) && t.gpos[0] >= 0 &&
+               t.gpos[0] < 64 && ridx[t.gpos[0]] < 0;
+    };
+    auto blocker = [&](const TileOp &t) {
+        if (t.flags & TF_DOT) return true;
+        return !diagonal_kind(t.kind) && t.kind != TOP_DENSE1 && t.kind != TOP_DENSE2;
+    };
+    int count = 0;
+    for (const TileOp &t : ops) count += is_tphase(t);
+    if (count < 2) return;
+    std::vector<TileOp> out;
+    std::vector<std::pair<uint32_t, size_t>> open; // (state mask, index into out)
+    for (const TileOp &t : ops) {
+        if (blocker(t)) open.clear();
+        if (!is_tphase(t)) {
+            out.push_back(t);
+            continue;
+        }
+        size_t at = out.size();
+        bool found = false;
+        for (const auto &g : open)
+            if (g.first == t.smask && out[g.second].k < static_cast<uint32_t>(kMaxTileOpTargets)) {
+                at = g.second;
+                found = true;
+            }
+        if (!found) {
+            TileOp n{};
+            n.kind = TOP_TPHASE;
+            n.smask = t.smask;
+            n.k = 0;
+            for (int j = 0; j < kMaxTileOpTargets; ++j) n.lpos[j] = n.gpos[j] = -1;
+            n.coef_off = p.coef.size();
+            // room for a full op: coefficients of one op are contiguous
+            p.coef.insert(p.coef.end(), 2 * kMaxTileOpTargets, cd(1.0));
+            p.coef_src.insert(p.coef_src.end(), 2 * kMaxTileOpTargets, PassPlan::CoefSrc{PassPlan::kNoSrc, 0});
+            out.push_back(n);
+            open.emplace_back(t.smask, at);
+        }
+        TileOp &g = out[at];
+        const uint64_t dst = g.coef_off + 2 * static_cast<uint64_t>(g.k);
+        for (int c = 0; c < 2; ++c) {
+            p.coef[dst + c] = p.coef[t.coef_off + c];
+            p.coef_src[dst + c] = p.coef_src[t.coef_off + c];
+        }
+        g.gpos[g.k] = t.gpos[0];
+        ++g.k;
+    }
+    ops.swap(out);
+}
+
 // In each register stage, one-bit phases on register bits commute with
 // diagonal ops and with dense ops on other register bits, so each can sit
 // anywhere between the ops that block it (a dense op on its bit, a dot, any
@@ -561,6 +620,7 @@ void group_register_phases(PassPlan &p, int rb) {
             int ridx[64];
             for (int b = 0; b < 64; ++b) ridx[b] = -1;
             for (int i = 0; i < rb; ++i) ridx[p.tbits[st.rbits[i]]] = i;
+            group_thread_phases(p, ops, ridx);
             auto is_phase = [&](const TileOp &t) {
                 return t.kind == TOP_DIAG && t.k == 1 && t.gcmask == 0 && !(t.flags & TF_DOT) && t.gpos[0] >= 0 &&
                        t.gpos[0] < 64 && ridx[t.gpos[0]] >= 0;

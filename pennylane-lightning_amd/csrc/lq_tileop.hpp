@@ -32,6 +32,8 @@ enum TileOpKind : uint32_t {
     TOP_SIGN = 8,    // negate where (index & gcmask) == gcval  (Z, CZ, multi-controlled Z)
     TOP_CZSET = 9,   // negate where sum_i popcnt(index & (index >> sd[i]) & sm[i]) is odd
     TOP_RPHASE = 10, // register stages: amplitude at register index e times coef[e] (2^RB table)
+    TOP_TPHASE = 11, // register stages: every amplitude times prod_j coef[2j + bit gpos[j] of the
+                     // index], over k bits fixed per thread (one-bit phases off the register bits)
 };
 
 // Register-resident tile programs (tile_reg_kernel): a pass is a list of
