@@ -198,11 +198,22 @@ class Gen {
             o << "  const unsigned long long bid = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * gridDim.x;\n";
             o << "  if (bid >= ntiles) return;\n";
             o << "  const unsigned long long tile = ntiles >= 8ull ? (bid & 7ull) * (ntiles >> 3) + (bid >> 3) : bid;\n";
+            // load straight into the first stage's mapping when its thread
+            // bits start with the contiguous low bits (the loads coalesce as
+            // in the default mapping): no entry transpose
+            Mapping lm = m0;
+            const TileStage &s0 = p_.stages.front();
+            if (s0.kind == STAGE_REG && s0.entry == ENTRY_TRANSPOSE) {
+                Mapping m{};
+                for (int i = 0; i < rb_; ++i) m.r[i] = s0.rbits[i];
+                for (int j = 0; j < 8; ++j) m.h[j] = s0.hbits[j];
+                if (coalesces(m)) lm = m;
+            }
             o << "  {\n";
-            emit_base_gl("tile", "gl0", m0);
-            emit_loads("v", "gl0", m0);
+            emit_base_gl("tile", "gl0", lm);
+            emit_loads("v", "gl0", lm);
             o << "  }\n";
-            emit_tile_body(m0, nslots);
+            emit_tile_body(m0, nslots, lm);
         } else {
             // persistent workgroups: each walks tiles blockIdx.x + k * gridDim.x
             // and loads the next tile while it works on the current one
@@ -223,7 +234,7 @@ class Gen {
             emit_base_gl("ltile", "gln", m0);
             emit_loads("n", "gln", m0);
             o << "  }\n";
-            emit_tile_body(m0, nslots);
+            emit_tile_body(m0, nslots, m0);
             o << "  if (!more) break;\n";
             o << "  tile = ntile;\n";
             for (int s = 0; s < ns_; ++s) {
@@ -264,8 +275,24 @@ class Gen {
             }
         }
     }
-    // stages, store and dot partials of the tile `tile` held in v*
-    void emit_tile_body(const Mapping &m0, int nslots) {
+    // thread bits of m begin with the tile's contiguous low bits (the first
+    // 16 complex128 / 32 complex64 amplitudes of a run are held by consecutive lanes)
+    bool coalesces(const Mapping &m) const {
+        const int L = dtype_ ? 4 : 5;
+        if (geom_.c < L) return false;
+        for (int j = 0; j < L; ++j)
+            if (m.h[j] != j) return false;
+        return true;
+    }
+    static bool same_mapping(const Mapping &a, const Mapping &b) {
+        for (int j = 0; j < 8; ++j)
+            if (a.h[j] != b.h[j]) return false;
+        for (int i = 0; i < kMaxRegBits; ++i)
+            if (a.r[i] != b.r[i]) return false;
+        return true;
+    }
+    // stages, store and dot partials of the tile `tile` held in v* (loaded in mapping lm)
+    void emit_tile_body(const Mapping &m0, int nslots, const Mapping &lm) {
         std::ostringstream &o = o_;
         o << "  {\n";
         o << "  const unsigned long long base = " << deposit("tile", geom_.ob, geom_.nout, true) << ";\n";
@@ -274,14 +301,14 @@ class Gen {
             o << "  for (unsigned i = tid; i < 128u; i += 256u) dred[i >> 5][i & 31u] = make_double2(0.0, 0.0);\n";
             o << "  __syncthreads();\n";
         }
-        cur_ = m0;
+        cur_ = lm;
         emit_gl("gl0", m0);
         for (size_t si = 0; si < p_.stages.size(); ++si) {
             const TileStage &st = p_.stages[si];
             Mapping m{};
             for (int i = 0; i < rb_; ++i) m.r[i] = st.rbits[i];
             for (int j = 0; j < 8; ++j) m.h[j] = st.hbits[j];
-            if (st.entry == ENTRY_TRANSPOSE) transpose(m);
+            if (st.entry == ENTRY_TRANSPOSE && !same_mapping(cur_, m)) transpose(m);
             cur_ = m;
             o << "  {\n";
             emit_gl("gt", m);
@@ -289,15 +316,22 @@ class Gen {
             o << "  }\n";
         }
         if (p_.store_mask) {
-            if (p_.end_transpose) transpose(m0);
-            cur_ = m0;
+            // store straight from the last stage's mapping when it coalesces
+            const char *gs = "gl0";
+            if (p_.end_transpose && !persist_ && coalesces(cur_)) {
+                emit_gl("gls", cur_);
+                gs = "gls";
+            } else {
+                if (p_.end_transpose) transpose(m0);
+                cur_ = m0;
+            }
             for (int s = 0; s < ns_; ++s) {
                 if (!((p_.store_mask >> s) & 1u)) continue;
                 for (int e = 0; e < E_; ++e) {
                     if (nt_)
-                        o << "  stnt(&sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "], v" << s << "[" << e << "]);\n";
+                        o << "  stnt(&sv" << s << "[" << gs << " | " << hex64(elem_global(cur_, e)) << "], v" << s << "[" << e << "]);\n";
                     else
-                        o << "  sv" << s << "[gl0 | " << hex64(elem_global(m0, e)) << "] = v" << s << "[" << e << "];\n";
+                        o << "  sv" << s << "[" << gs << " | " << hex64(elem_global(cur_, e)) << "] = v" << s << "[" << e << "];\n";
                 }
             }
         }
