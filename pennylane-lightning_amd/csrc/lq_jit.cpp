@@ -277,12 +277,21 @@ class Gen {
     }
     // thread bits of m begin with the tile's contiguous low bits (the first
     // 16 complex128 / 32 complex64 amplitudes of a run are held by consecutive lanes)
+    //
+    // Taking a stage's mapping for the loads or stores saves an LDS transpose
+    // but can spread each memory instruction over farther segments; measured
+    // on the 30-qubit passes it pays only in passes that would otherwise make
+    // four or more transposes (memory-bound lighter passes got slower).
     bool coalesces(const Mapping &m) const {
         const int L = dtype_ ? 4 : 5;
         if (geom_.c < L) return false;
         for (int j = 0; j < L; ++j)
             if (m.h[j] != j) return false;
-        return true;
+        const int mode = jit_config().direct_io;
+        if (mode != 1) return mode == 2;
+        int nt = p_.end_transpose ? 1 : 0;
+        for (const auto &st : p_.stages) nt += st.entry == ENTRY_TRANSPOSE;
+        return nt >= 4;
     }
     static bool same_mapping(const Mapping &a, const Mapping &b) {
         for (int j = 0; j < 8; ++j)
@@ -819,6 +828,7 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_LDS32")) j.lds32 = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_SCALE")) j.scale_rotations = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_WAVES")) j.waves_per_eu = std::atoi(s);
+        if (const char *s = std::getenv("LQ_JIT_DIRECT")) j.direct_io = std::atoi(s);
         return j;
     }();
     return c;

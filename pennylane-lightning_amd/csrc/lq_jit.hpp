@@ -33,6 +33,8 @@ struct JitConfig {
                               // so that more workgroups fit per CU
     int waves_per_eu = -1;    // > 0: ask the compiler to fit that many waves per SIMD; 0: no
                               // request; -1: by kind (one-state passes: 3 complex128, 4 complex64)
+    int direct_io = 1;        // loads/stores in the first/last stage's mapping (no entry/exit
+                              // transpose): 0 never, 1 where measured to pay, 2 whenever they coalesce
 };
 JitConfig &jit_config();
 // Waves per SIMD requested of a specialised pass kernel (0: none).
