@@ -108,8 +108,9 @@ int lq_sv_synchronize(lq_sv_t sv) {
 
 int lq_sv_device_ptr(lq_sv_t sv, void **out) {
     return guard([&] {
-        S(sv).synchronize();
+        // data() first: it may enqueue the write of a deferred reset
         *out = S(sv).data();
+        S(sv).synchronize();
     });
 }
 

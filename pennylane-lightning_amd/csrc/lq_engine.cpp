@@ -321,6 +321,15 @@ void StateVector::reset() { set_basis_index(0); }
 void StateVector::set_basis_index(uint64_t index) {
     LQ_ABORT_IF(index > length() - 1, "Invalid index");
     queue_.clear();
+    // deferred: the next program's first specialised pass forms the state,
+    // any other use writes it first
+    pending_basis_ = static_cast<int64_t>(index);
+}
+
+void StateVector::materialize() {
+    if (pending_basis_ < 0) return;
+    const uint64_t index = static_cast<uint64_t>(pending_basis_);
+    pending_basis_ = -1;
     dispatch(dtype_, [&](auto t) { launch_fill_basis<decltype(t)>(d_, n_, index, stream_); });
 }
 
@@ -341,13 +350,14 @@ void StateVector::set_state_vector(const void *host, const uint32_t *wires, int 
     hip_check(hipMemcpyAsync(small_.p, host, b, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync");
     std::vector<int> bits(nw);
     for (int j = 0; j < nw; ++j) bits[j] = phys_bit(n_, wires[j]);
-    dispatch(dtype_, [&](auto t) { launch_set_subspace<decltype(t)>(d_, n_, small_.p, bits.data(), nw, stream_); });
+    dispatch(dtype_, [&](auto t) { launch_set_subspace<decltype(t)>(data(), n_, small_.p, bits.data(), nw, stream_); });
     synchronize();
 }
 
 void StateVector::update_data(const void *host, uint64_t len) {
     LQ_ABORT_IF(len != length(), "The size of provided data must be equal to the statevector size.");
     queue_.clear();
+    pending_basis_ = -1; // every amplitude is overwritten
     hip_check(hipMemcpyAsync(d_, host, bytes(), hipMemcpyHostToDevice, stream_), "hipMemcpyAsync");
     synchronize();
 }
@@ -355,7 +365,7 @@ void StateVector::update_data(const void *host, uint64_t len) {
 void StateVector::get_data(void *host, uint64_t len) {
     LQ_ABORT_IF(len != length(), "The size of provided data must be equal to the statevector size.");
     flush();
-    hip_check(hipMemcpyAsync(host, d_, bytes(), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(host, data(), bytes(), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
     synchronize();
 }
 
@@ -364,7 +374,8 @@ void StateVector::copy_from(StateVector &src) {
     src.flush();
     queue_.clear();
     src.synchronize();
-    hip_check(hipMemcpyAsync(d_, src.d_, bytes(), hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync");
+    pending_basis_ = -1; // every amplitude is overwritten
+    hip_check(hipMemcpyAsync(d_, src.data(), bytes(), hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync");
 }
 
 void StateVector::enqueue(LOp op) {
@@ -540,7 +551,7 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         st_off[p] = st_bytes;
         st_bytes += passes[p].stages.size() * sizeof(TileStage);
         coef_off[p] = coef_n;
-        coef_n += passes[p].coef.size();
+        coef_n += passes[p].coef.size() + 1; // + the deferred-reset entry read by specialised passes
         part_off[p] = part_n;
         const uint64_t ntiles = 1ull << (n_ - static_cast<int>(passes[p].tbits.size()));
         for (size_t s = 0; s < passes[p].slot_ids.size(); ++s) {
@@ -556,6 +567,19 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
     const size_t ranges_at = align(coef_at + coef_n * sizeof(double2));
     const size_t total = align(ranges_at + ranges.size() * sizeof(SlotRange));
 
+    // A deferred reset of this state is formed by the first pass when that is
+    // a specialised one-state pass reading it; otherwise it is written now.
+    bool form_basis = false;
+    if (pending_basis_ >= 0) {
+        bool reads = false;
+        for (void *sp : states) reads |= sp == d_;
+        if (reads) {
+            form_basis = ns == 1 && fns[0] != nullptr && passes[0].reg && (passes[0].load_mask & 1u) &&
+                         !jit_persistent(dtype_);
+            if (!form_basis) materialize();
+        }
+    }
+
     // the staging buffer may still feed the previous upload
     hip_check(hipEventSynchronize(staged_), "hipEventSynchronize");
     staging_.ensure(total);
@@ -567,7 +591,10 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         double2 *c = reinterpret_cast<double2 *>(hs + coef_at) + coef_off[p];
         for (size_t i = 0; i < passes[p].coef.size(); ++i)
             c[i] = make_double2(passes[p].coef[i].real(), passes[p].coef[i].imag());
+        c[passes[p].coef.size()] = (p == 0 && form_basis) ? make_double2(1.0, static_cast<double>(pending_basis_))
+                                                          : make_double2(-1.0, 0.0);
     }
+    if (form_basis) pending_basis_ = -1;
     if (!ranges.empty()) std::memcpy(hs + ranges_at, ranges.data(), ranges.size() * sizeof(SlotRange));
     prog_.ensure(total);
     hip_check(hipMemcpyAsync(prog_.p, hs, total, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
@@ -776,7 +803,7 @@ double StateVector::norm2() {
     const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
     partials_.ensure(nb * sizeof(double2));
     dispatch(dtype_, [&](auto t) {
-        launch_norm2<decltype(t)>(d_, n_, static_cast<double2 *>(partials_.p), nb, stream_);
+        launch_norm2<decltype(t)>(data(), n_, static_cast<double2 *>(partials_.p), nb, stream_);
     });
     std::vector<double2> h(nb);
     hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
@@ -796,7 +823,7 @@ cd StateVector::overlap(StateVector &other) {
     other.synchronize();
     const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
     partials_.ensure(nb * sizeof(double2));
-    launch_overlap(d_, dtype_, other.d_, other.dtype_, n_, static_cast<double2 *>(partials_.p), nb, stream_);
+    launch_overlap(data(), dtype_, other.data(), other.dtype_, n_, static_cast<double2 *>(partials_.p), nb, stream_);
     std::vector<double2> h(nb);
     hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
     hip_check(hipStreamSynchronize(stream_), "sync");
@@ -816,14 +843,14 @@ void StateVector::normalize() {
     const double norm = std::sqrt(norm2());
     const double eps = dtype_ ? std::numeric_limits<double>::epsilon() : std::numeric_limits<float>::epsilon();
     LQ_ABORT_IF(norm < eps * 1e2, "Vector has norm close to zero and cannot be normalized");
-    dispatch(dtype_, [&](auto t) { launch_scale<decltype(t)>(d_, n_, 1.0 / norm, 0.0, stream_); });
+    dispatch(dtype_, [&](auto t) { launch_scale<decltype(t)>(data(), n_, 1.0 / norm, 0.0, stream_); });
 }
 
 void StateVector::collapse(uint32_t wire, bool branch) {
     LQ_ABORT_IF(static_cast<int>(wire) >= n_, "wire index out of range");
     flush();
     dispatch(dtype_, [&](auto t) {
-        launch_collapse<decltype(t)>(d_, n_, phys_bit(n_, wire), branch ? 1 : 0, 1.0, stream_);
+        launch_collapse<decltype(t)>(data(), n_, phys_bit(n_, wire), branch ? 1 : 0, 1.0, stream_);
     });
     normalize();
 }
@@ -851,7 +878,7 @@ std::vector<double> StateVector::probs(const uint32_t *wires, int nw, bool all) 
     if (m > 12) {
         slots_.ensure(sizeof(double) << m);
         dispatch(dtype_, [&](auto t) {
-            launch_probs_gather<decltype(t)>(d_, n_, bits.data(), m, static_cast<double *>(slots_.p), stream_);
+            launch_probs_gather<decltype(t)>(data(), n_, bits.data(), m, static_cast<double *>(slots_.p), stream_);
         });
         std::vector<double> h(static_cast<size_t>(1) << m);
         hip_check(hipMemcpyAsync(h.data(), slots_.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_),
@@ -862,7 +889,7 @@ std::vector<double> StateVector::probs(const uint32_t *wires, int nw, bool all) 
     const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(512, std::max<uint64_t>(1, length() / 1024)));
     partials_.ensure((static_cast<size_t>(nb) * sizeof(double)) << m);
     dispatch(dtype_, [&](auto t) {
-        launch_probs_hist<decltype(t)>(d_, n_, bits.data(), m, static_cast<double *>(partials_.p), nb, stream_);
+        launch_probs_hist<decltype(t)>(data(), n_, bits.data(), m, static_cast<double *>(partials_.p), nb, stream_);
     });
     return reduce_rows_to_host(nb, 1u << m);
 }
@@ -1053,7 +1080,7 @@ double StateVector::expval_obs(const Obs &o) {
         const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
         partials_.ensure(nb * sizeof(double2));
         dispatch(dtype_, [&](auto t) {
-            launch_pauli_sum<decltype(t)>(d_, nullptr, static_cast<const PauliTerm *>(small_.p),
+            launch_pauli_sum<decltype(t)>(data(), nullptr, static_cast<const PauliTerm *>(small_.p),
                                           static_cast<uint32_t>(terms.size()), n_, 0,
                                           static_cast<double2 *>(partials_.p), nb, stream_);
         });
@@ -1069,8 +1096,8 @@ double StateVector::expval_obs(const Obs &o) {
         return s + c;
     }
     void *tmp = workspace(6);
-    apply_obs_to(o, d_, tmp);
-    return inner(d_, tmp).real();
+    apply_obs_to(o, data(), tmp);
+    return inner(data(), tmp).real();
 }
 
 // ---------------------------------------------------------------------------
@@ -1154,7 +1181,7 @@ double StateVector::var_obs(const Obs &o) {
     }
     flush();
     void *tmp = workspace(6);
-    apply_obs_to(o, d_, tmp);
+    apply_obs_to(o, data(), tmp);
     // <O psi|O psi> and Re<O psi|psi> in one read pass
     const auto v = run({identity_dot(1, 1, 0), identity_dot(0, 1, 1)}, {d_, tmp}, 2);
     const double mean = v[1].real();
@@ -1183,7 +1210,7 @@ void StateVector::sparse_moments(const CsrView &h, double *mean, double *mean_sq
     const auto *indices = reinterpret_cast<const int64_t *>(base + csr_indices_off_);
     const void *values = base + csr_values_off_;
     dispatch(dtype_, [&](auto t) {
-        launch_sparse_rows<decltype(t)>(d_, nullptr, indptr, indices, values, h.nrows, 0,
+        launch_sparse_rows<decltype(t)>(data(), nullptr, indptr, indices, values, h.nrows, 0,
                                         static_cast<double2 *>(partials_.p), nb, stream_);
     });
     std::vector<double2> hp(2 * static_cast<size_t>(nb));

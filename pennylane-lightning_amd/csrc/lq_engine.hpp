@@ -110,7 +110,11 @@ class StateVector {
     size_t elem() const { return dtype_ ? 16 : 8; }
     uint64_t length() const { return 1ull << n_; }
     size_t bytes() const { return elem() << n_; }
-    void *data() { return d_; }
+    // device state; a deferred reset is written first
+    void *data() {
+        materialize();
+        return d_;
+    }
     hipStream_t stream() const { return stream_; }
 
     // state preparation (lq/StateVectorLQubit.hpp:928-1036)
@@ -201,6 +205,11 @@ class StateVector {
   private:
     int n_, dtype_;
     void *d_ = nullptr;
+    // reset deferred: the state is the basis state |pending_basis_> but not yet
+    // written; the first specialised pass of the next program forms it, any
+    // other use writes it first (materialize)
+    int64_t pending_basis_ = -1;
+    void materialize();
     hipStream_t stream_ = nullptr;
     std::vector<LOp> queue_;
     DevBuf prog_, partials_, slots_, small_;

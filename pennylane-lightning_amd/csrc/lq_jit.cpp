@@ -211,7 +211,21 @@ class Gen {
             }
             o << "  {\n";
             emit_base_gl("tile", "gl0", lm);
-            emit_loads("v", "gl0", lm);
+            if (ns_ == 1) {
+                // the entry after the pass's coefficients: (1, k) when the state
+                // is the basis state |k> not yet written (a deferred reset); the
+                // pass then forms its input instead of reading it
+                o << "  const double2 bmark = ((const kcoef *)coef)[" << p_.coef.size() << "];\n";
+                o << "  if (bmark.x > 0.0) {\n";
+                o << "    const unsigned long long bidx = (unsigned long long)bmark.y;\n";
+                for (int e = 0; e < E_; ++e)
+                    o << "    v0[" << e << "] = mk((gl0_ | " << hex64(elem_global(lm, e)) << ") == bidx ? (real)1 : (real)0, (real)0);\n";
+                o << "  } else {\n";
+                emit_loads("v", "gl0", lm);
+                o << "  }\n";
+            } else {
+                emit_loads("v", "gl0", lm);
+            }
             o << "  }\n";
             emit_tile_body(m0, nslots, lm);
         } else {

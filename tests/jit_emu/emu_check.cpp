@@ -145,6 +145,13 @@ int main(int argc, char **argv) {
     // initial states: random normalised
     std::vector<St> S(ns, St(1ull << n));
     for (auto &s : S) for (auto &x : s) x = cd(U(rng) - 3.14, U(rng) - 3.14);
+    // EMU_BASIS=k (one state): the state is the basis state |k> not yet
+    // written (a deferred reset); the first pass must form it and ignore memory
+    const long long basis = (getenv("EMU_BASIS") && ns == 1) ? atoll(getenv("EMU_BASIS")) : -1;
+    if (basis >= 0) {
+        for (auto &x : S[0]) x = 0.0;
+        S[0][static_cast<size_t>(basis)] = 1.0;
+    }
     std::vector<St> R = S;
     std::vector<cd> rdots(nslots), edots(nslots);
     for (const auto &o : prog) ref_apply(R, n, o, rdots);
@@ -153,8 +160,10 @@ int main(int argc, char **argv) {
     std::vector<std::vector<char>> buf(2, std::vector<char>(esz << n));
     for (int s = 0; s < ns; ++s)
         for (uint64_t i = 0; i < (1ull << n); ++i) {
-            if (dtype) { double *p = (double *)buf[s].data(); p[2 * i] = S[s][i].real(); p[2 * i + 1] = S[s][i].imag(); }
-            else { float *p = (float *)buf[s].data(); p[2 * i] = (float)S[s][i].real(); p[2 * i + 1] = (float)S[s][i].imag(); }
+            // with a deferred reset the memory holds unrelated values
+            const cd v = basis >= 0 ? cd(U(rng), -U(rng)) : S[s][i];
+            if (dtype) { double *p = (double *)buf[s].data(); p[2 * i] = v.real(); p[2 * i + 1] = v.imag(); }
+            else { float *p = (float *)buf[s].data(); p[2 * i] = (float)v.real(); p[2 * i + 1] = (float)v.imag(); }
         }
     for (size_t p = 0; p < passes.size(); ++p) {
         const std::string src = jit_pass_source(passes[p], n, dtype, ns);
@@ -167,8 +176,10 @@ int main(int argc, char **argv) {
         void *h = dlopen((base + ".so").c_str(), RTLD_NOW | RTLD_LOCAL);
         if (!h) { printf("dlopen: %s\n", dlerror()); return 4; }
         Launch L = (Launch)dlsym(h, "emu_launch");
-        std::vector<double2> coef(passes[p].coef.size());
-        for (size_t i = 0; i < coef.size(); ++i) coef[i] = {passes[p].coef[i].real(), passes[p].coef[i].imag()};
+        std::vector<double2> coef(passes[p].coef.size() + 1);
+        for (size_t i = 0; i + 1 < coef.size(); ++i) coef[i] = {passes[p].coef[i].real(), passes[p].coef[i].imag()};
+        // the entry after the coefficients: deferred reset marker, as the engine writes it
+        coef.back() = (p == 0 && basis >= 0) ? double2{1.0, static_cast<double>(basis)} : double2{-1.0, 0.0};
         const uint64_t ntiles = 1ull << (n - (int)passes[p].tbits.size());
         std::vector<double2> part(std::max<size_t>(1, passes[p].slot_ids.size() * ntiles), {0, 0});
         const unsigned grid = jit_config().persistent ? (unsigned)std::min<uint64_t>(ntiles, grid_cap) : (unsigned)ntiles;

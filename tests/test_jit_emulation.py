@@ -36,15 +36,18 @@ def emu_binary(tmp_path_factory):
 @pytest.mark.parametrize("case,dtype,persistent,lds32,look", [
     (0, 1, 0, 0, 1), (1, 1, 0, 0, 1), (2, 1, 0, 0, 1), (1, 0, 0, 0, 1), (2, 0, 0, 0, 1), (2, 1, 1, 0, 1),
     (0, 1, 0, 1, 1), (2, 1, 0, 1, 1), (2, 0, 0, 1, 1), (0, 1, 0, 0, 0), (1, 0, 0, 0, 0), (2, 1, 0, 0, 0),
-    (0, 1, 0, 1, 2), (1, 0, 0, 1, 2), (2, 1, 0, 1, 2)])
+    (0, 1, 0, 1, 2), (1, 0, 0, 1, 2), (2, 1, 0, 1, 2), (0, 1, 0, 1, 3), (1, 0, 0, 1, 3)])
 def test_generated_kernels_match_naive_application(emu_binary, case, dtype, persistent, lds32, look):
     exe, work = emu_binary
     # lds32: transposes through 32 KiB of LDS (emulated at exactly the size the kernel declares);
     # look 0: tile bits from the first ready op instead of the planner's lookahead;
     # look 2: lookahead, and loads/stores in the first/last stage's mapping wherever they coalesce
     # (16 qubits: passes whose first and last stages leave the low bits to the lanes)
+    # look 3: lookahead, and the state a basis state not yet written (deferred reset, formed by the first pass)
     env = dict(os.environ, LQ_JIT_PERSIST=str(persistent), LQ_JIT_LDS32=str(lds32), EMU_LOOKAHEAD=str(min(look, 1)),
                LQ_JIT_DIRECT="2" if look == 2 else "1")
+    if look == 3:
+        env["EMU_BASIS"] = "5"
     # 14 qubits: 4 tiles per pass, walked by 3 workgroups in the persistent form
     r = subprocess.run([exe, str(case), str(dtype), "16" if look == 2 else "14", "3", EMU, work], capture_output=True, text=True,
                        timeout=600, env=env)
