@@ -639,8 +639,9 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         if (prof_) {
             pe.a = take_event();
             pe.b = take_event();
-            pe.bytes = static_cast<double>(bytes()) *
-                       (__builtin_popcount(pp.load_mask) + __builtin_popcount(pp.store_mask));
+            // a pass that forms a deferred reset reads nothing of that state
+            const int reads = __builtin_popcount(pp.load_mask) - ((p == 0 && form_basis) ? 1 : 0);
+            pe.bytes = static_cast<double>(bytes()) * (reads + __builtin_popcount(pp.store_mask));
             hip_check(hipEventRecord(pe.a, stream_), "hipEventRecord");
         }
         if (fns[p])
