@@ -372,10 +372,13 @@ void StateVector::get_data(void *host, uint64_t len) {
 void StateVector::copy_from(StateVector &src) {
     LQ_ABORT_IF(src.n_ != n_ || src.dtype_ != dtype_, "state vectors differ in size or precision");
     src.flush();
-    queue_.clear();
+    // data() may enqueue the write of a deferred reset on src's stream: take
+    // it before waiting for that stream
+    const void *sp = src.data();
     src.synchronize();
+    queue_.clear();
     pending_basis_ = -1; // every amplitude is overwritten
-    hip_check(hipMemcpyAsync(d_, src.data(), bytes(), hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(d_, sp, bytes(), hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync");
 }
 
 void StateVector::enqueue(LOp op) {
@@ -821,10 +824,12 @@ double StateVector::norm2() {
 cd StateVector::overlap(StateVector &other) {
     LQ_ABORT_IF(other.n_ != n_, "states of different sizes");
     flush();
+    other.flush();
+    const void *op = other.data(); // before the wait: it may enqueue a deferred reset on other's stream
     other.synchronize();
     const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
     partials_.ensure(nb * sizeof(double2));
-    launch_overlap(data(), dtype_, other.data(), other.dtype_, n_, static_cast<double2 *>(partials_.p), nb, stream_);
+    launch_overlap(data(), dtype_, op, other.dtype_, n_, static_cast<double2 *>(partials_.p), nb, stream_);
     std::vector<double2> h(nb);
     hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
     hip_check(hipStreamSynchronize(stream_), "sync");
