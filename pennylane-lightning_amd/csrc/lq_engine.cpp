@@ -485,9 +485,11 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
     };
     if (use_jit && jcfg.split_rotations) {
         PlanConfig scfg = cfg;
-        // measured on the 30-qubit HEA: complex128 48 (40: 140 ms, 56: 127 ms,
-        // 64: 136 ms against 129), complex64 40 (36: 89 ms, 44: 94 ms against 86)
-        if (t.budget <= 0) scfg.budget = dtype_ ? 48.0 : 40.0;
+        // measured on the 30-qubit HEA: complex128 52 (18 passes, 122 ms; 48
+        // and 54: 19 passes, 126 and 131 ms; the 24-qubit adjoint and 33-qubit
+        // QFT workloads equal or faster), complex64 40 (36: 89 ms, 44: 94 ms
+        // against 86)
+        if (t.budget <= 0) scfg.budget = dtype_ ? 52.0 : 40.0;
         scfg.scale_rotations = jcfg.scale_rotations;
         // complex64: keep the unscaled amplitudes (at most 1) well inside float range
         scfg.max_scale_growth = dtype_ ? 1e300 : 1e20;

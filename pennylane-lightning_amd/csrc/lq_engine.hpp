@@ -89,7 +89,7 @@ struct OpsList {
 struct EngineTuning {
     int kt_max = 0;      // 0: from the LDS budget
     int low_bits = -1;   // -1: 256-byte segments
-    double budget = 0.0; // 0: 24 per pass; specialised passes with split rotations 48 (complex64 40)
+    double budget = 0.0; // 0: 24 per pass; specialised passes with split rotations 52 (complex64 40)
     size_t lds_bytes = 65536;
     bool reg = true;     // register-resident passes for large tiles
     bool plan_debug = false;

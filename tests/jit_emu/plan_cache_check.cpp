@@ -79,7 +79,7 @@ int main() {
                     cfg.ns = ns;
                     cfg.kt_max = 12;
                     cfg.low_bits = 4;
-                    cfg.budget = split ? 48 : 24;
+                    cfg.budget = split ? 52 : 24;
                     cfg.lookahead = split != 0; // as the engine plans specialised passes
                     cfg.reg_rb = 4;
                     cfg.max_dots = 32;
