@@ -39,7 +39,7 @@ HBM_PEAK = 8.0e12
 # HBM bytes per specialised pass launch measured with rocprofv3 (FETCH_SIZE and
 # WRITE_SIZE in separate runs of this script, each scaled by the same counter
 # over torch.neg of a known 2 GiB in / 2 GiB out): (precision, qubits) -> bytes
-MEASURED_TRAFFIC = {("c128", 30): 32553776176}
+MEASURED_TRAFFIC = {("c128", 30): 32453777030}
 
 
 def gate_bytes(op, n, elem):
