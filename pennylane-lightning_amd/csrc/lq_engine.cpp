@@ -575,14 +575,15 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
     const size_t total = align(ranges_at + ranges.size() * sizeof(SlotRange));
 
     // A deferred reset of this state is formed by the first pass when that is
-    // a specialised one-state pass reading it; otherwise it is written now.
+    // a specialised one-state pass reading and writing it (a read-only pass
+    // would leave the memory unwritten); otherwise it is written now.
     bool form_basis = false;
     if (pending_basis_ >= 0) {
         bool reads = false;
         for (void *sp : states) reads |= sp == d_;
         if (reads) {
             form_basis = ns == 1 && fns[0] != nullptr && passes[0].reg && (passes[0].load_mask & 1u) &&
-                         !jit_persistent(dtype_);
+                         (passes[0].store_mask & 1u) && !jit_persistent(dtype_);
             if (!form_basis) materialize();
         }
     }

@@ -108,3 +108,23 @@ def test_reset_then_matrix_and_adjoint():
     from helpers import ham_to_oracle, to_oracle_ops
     jexp = np.asarray(O.adjoint_jacobian(st, N, to_oracle_ops(ops), [ham_to_oracle(coeffs, words)], tp)).reshape(-1)
     np.testing.assert_allclose(jac, jexp, atol=1e-10)
+
+
+def test_reset_then_read_only_pass_leaves_state_written():
+    # an expval right after a reset runs a read-only dot pass; once its
+    # specialised kernel exists, the reset must still be written to memory
+    bits = [1, 1, 0, 1] + [0] * (N - 4)
+    exp = basis_state(N, bits)
+    sv = scrambled(N, 11)
+    m = L.MeasurementsC128(sv)
+    counts = []
+    for attempt in range(2):
+        sv.updateData(rand_state(N, np.random.default_rng(20 + attempt)))
+        sv.setBasisState(bits, list(range(N)))
+        sv.pass_counts(reset=True)
+        assert m.expval("PauliX", [5]) == pytest.approx(0.0, abs=TOL_C128)
+        counts.append(sv.pass_counts(reset=True))
+        np.testing.assert_allclose(sv.state(), exp, atol=0)
+        if attempt == 0:
+            L.jit_wait()
+    assert counts[1]["specialised"] > 0, counts
