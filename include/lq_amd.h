@@ -236,11 +236,16 @@ int lq_server_stats(lq_server_t srv, uint64_t *out6);
  * generic kernels. lq_jit_shutdown drops queued compiles and waits for those
  * in flight (no new compiles start afterwards); call it before the process
  * exits when compiles may be in flight -- the Python package does so from
- * its exit hook. */
+ * its exit hook. lq_set_split_fallback chooses what runs while some
+ * specialised kernels of a program are missing: 0 the unsplit plan in the
+ * generic kernels, 1 (default, or LQ_SPLIT_FALLBACK) the cheaper of the two
+ * by an op-count estimate, 2 the split plan with its missing passes in the
+ * generic kernel; the previous mode is returned in *previous (may be NULL). */
 int lq_jit_wait(void);
 int lq_jit_shutdown(void);
 int lq_jit_stats(int *ready, int *failed, int *pending);
 int lq_sv_pass_counts(lq_sv_t sv, int reset, uint64_t *specialised, uint64_t *generic);
+int lq_set_split_fallback(int mode, int *previous);
 
 #ifdef __cplusplus
 }

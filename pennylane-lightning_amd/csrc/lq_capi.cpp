@@ -577,4 +577,12 @@ int lq_sv_pass_counts(lq_sv_t sv, int reset, uint64_t *specialised, uint64_t *ge
     return guard([&] { S(sv).pass_counts(reset != 0, specialised, generic); });
 }
 
+int lq_set_split_fallback(int mode, int *previous) {
+    return guard([&] {
+        LQ_ABORT_IF(mode < 0 || mode > 2, "split fallback mode must be 0, 1 or 2");
+        if (previous) *previous = tuning().split_fallback;
+        tuning().split_fallback = mode;
+    });
+}
+
 } // extern "C"

@@ -64,6 +64,7 @@ EngineTuning &tuning() {
         if (const char *s = std::getenv("LQ_PLAN_DEBUG")) e.plan_debug = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_PLAN_CACHE")) e.plan_cache = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_PLAN_LOOKAHEAD")) e.lookahead = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_SPLIT_FALLBACK")) e.split_fallback = std::atoi(s);
         return e;
     }();
     return t;
@@ -498,7 +499,7 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
         split_1q_rotations(split);
         passes = plan_cached(split, scfg);
         const size_t missing = lookup_all(fns);
-        if (missing) {
+        if (missing && t.split_fallback != 2) {
             // The generic kernel interprets the op list, so its pass time grows
             // with the ops: measured on the 30-qubit HEA passes, about 0.28 of
             // a specialised pass per op of the split plan (phase tables, real
@@ -509,7 +510,7 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
             for (size_t p = 0; p < passes.size(); ++p)
                 split_cost += (fns[p] || !passes[p].reg) ? 1.0 : 0.28 * static_cast<double>(passes[p].ops.size());
             for (const PassPlan &pp : plain) plain_cost += std::max(1.0, 0.16 * static_cast<double>(pp.ops.size()));
-            if (plain_cost < split_cost) {
+            if (t.split_fallback == 0 || plain_cost < split_cost) {
                 passes.swap(plain);
                 fns.assign(passes.size(), nullptr);
             }

@@ -95,6 +95,10 @@ struct EngineTuning {
     bool plan_debug = false;
     bool plan_cache = true; // reuse plans of programs with the same structure
     bool lookahead = true; // lookahead tile-bit choice for specialised passes
+    // while some specialised kernels of a split plan are missing: 0 run the
+    // unsplit plan, 1 choose by the op-count estimate, 2 run the split plan
+    // (its missing passes in the generic kernel)
+    int split_fallback = 1;
 };
 EngineTuning &tuning();
 

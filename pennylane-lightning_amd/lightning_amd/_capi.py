@@ -85,6 +85,7 @@ _SIGS = {
     "lq_jit_wait": (C.c_int, []),
     "lq_jit_shutdown": (C.c_int, []),
     "lq_jit_stats": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "lq_set_split_fallback": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     "lq_sv_pass_counts": (C.c_int, [_P, C.c_int, _U64P, _U64P]),
 }
 

@@ -64,6 +64,16 @@ def jit_wait():
     check(lib.lq_jit_wait())
 
 
+def set_split_fallback(mode: int) -> int:
+    """What runs while some specialised kernels of a program are missing:
+    0 the unsplit plan, 1 the cheaper by an op-count estimate (default),
+    2 the split plan with its missing passes in the generic kernel. Returns
+    the previous mode."""
+    prev = C.c_int()
+    check(lib.lq_set_split_fallback(int(mode), C.byref(prev)))
+    return prev.value
+
+
 def jit_stats() -> dict:
     """Specialised kernels loaded, compiles failed, compiles pending."""
     r, f, p = C.c_int(), C.c_int(), C.c_int()
