@@ -486,11 +486,14 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
     };
     if (use_jit && jcfg.split_rotations) {
         PlanConfig scfg = cfg;
-        // measured on the 30-qubit HEA: complex128 52 (18 passes, 122 ms; 48
-        // and 54: 19 passes, 126 and 131 ms; the 24-qubit adjoint and 33-qubit
-        // QFT workloads equal or faster), complex64 40 (36: 89 ms, 44: 94 ms
-        // against 86)
-        if (t.budget <= 0) scfg.budget = dtype_ ? 52.0 : 40.0;
+        // measured on the 30-qubit HEA: budget 52 (complex128 18 passes,
+        // 122 ms; 48 and 54: 19 passes, 126 and 131 ms; the 24-qubit adjoint
+        // and 33-qubit QFT workloads equal or faster), and in complex64 4
+        // contiguous low bits as in complex128 (one 128-byte line per 16
+        // lanes: 18 passes, 79-80 ms, against 24 passes and 86 ms with 5 low
+        // bits and budget 40)
+        if (t.budget <= 0) scfg.budget = 52.0;
+        if (t.low_bits < 0) scfg.low_bits = 4;
         scfg.scale_rotations = jcfg.scale_rotations;
         // complex64: keep the unscaled amplitudes (at most 1) well inside float range
         scfg.max_scale_growth = dtype_ ? 1e300 : 1e20;

@@ -88,8 +88,8 @@ struct OpsList {
 
 struct EngineTuning {
     int kt_max = 0;      // 0: from the LDS budget
-    int low_bits = -1;   // -1: 256-byte segments
-    double budget = 0.0; // 0: 24 per pass; specialised passes with split rotations 52 (complex64 40)
+    int low_bits = -1;   // -1: 4 (complex128) / 5 (complex64); specialised passes 4
+    double budget = 0.0; // 0: 24 per pass; specialised passes with split rotations 52
     size_t lds_bytes = 65536;
     bool reg = true;     // register-resident passes for large tiles
     bool plan_debug = false;

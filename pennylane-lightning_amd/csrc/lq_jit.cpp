@@ -289,15 +289,15 @@ class Gen {
             }
         }
     }
-    // thread bits of m begin with the tile's contiguous low bits (the first
-    // 16 complex128 / 32 complex64 amplitudes of a run are held by consecutive lanes)
+    // thread bits of m begin with the tile's 4 contiguous low bits (the 16
+    // amplitudes of a run are held by consecutive lanes)
     //
     // Taking a stage's mapping for the loads or stores saves an LDS transpose
     // but can spread each memory instruction over farther segments; measured
     // on the 30-qubit passes it pays only in passes that would otherwise make
     // four or more transposes (memory-bound lighter passes got slower).
     bool coalesces(const Mapping &m) const {
-        const int L = dtype_ ? 4 : 5;
+        const int L = 4; // 16 lanes: 256 bytes of complex128, one 128-byte line of complex64
         if (geom_.c < L) return false;
         for (int j = 0; j < L; ++j)
             if (m.h[j] != j) return false;

@@ -133,11 +133,11 @@ int main(int argc, char **argv) {
             dot(GateId::RY, 5, 0, ns - 1);
         }
     }
-    PlanConfig cfg; cfg.n = n; cfg.ns = ns; cfg.kt_max = 12; cfg.low_bits = getenv("EMU_LOW_BITS") ? atoi(getenv("EMU_LOW_BITS")) : (dtype ? 4 : 5); cfg.reg_rb = 4; cfg.max_dots = 32; cfg.scale_rotations = !getenv("EMU_NO_SCALE");
+    PlanConfig cfg; cfg.n = n; cfg.ns = ns; cfg.kt_max = 12; cfg.low_bits = getenv("EMU_LOW_BITS") ? atoi(getenv("EMU_LOW_BITS")) : (getenv("EMU_NO_SPLIT") ? (dtype ? 4 : 5) : 4); cfg.reg_rb = 4; cfg.max_dots = 32; cfg.scale_rotations = !getenv("EMU_NO_SCALE");
     cfg.max_scale_growth = dtype ? 1e300 : 1e20; // as the engine configures specialised passes
-    // the engine's specialised-pass planning: lookahead tile bits, budget 52 (complex64 40)
+    // the engine's specialised-pass planning: lookahead tile bits, budget 52, 4 low bits
     cfg.lookahead = !(getenv("EMU_LOOKAHEAD") && atoi(getenv("EMU_LOOKAHEAD")) == 0);
-    cfg.budget = getenv("EMU_NO_SPLIT") ? 24 : (dtype ? 52 : 40);
+    cfg.budget = getenv("EMU_NO_SPLIT") ? 24 : 52;
     // as the engine does for specialised passes: 2x2 gates as phase / real rotation / phase
     std::vector<LOp> split = prog;
     if (!getenv("EMU_NO_SPLIT")) split_1q_rotations(split);
