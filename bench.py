@@ -10,7 +10,14 @@ value = HBM % of peak over the whole job: the tile passes' algorithmic bytes
 (amplitudes read + written: 2 x 16 GiB per pass, 16 GiB for the first) over
 the wall time of the timed steps, against 8 TB/s. roofline.achieved is the
 same bytes over the passes' own kernel time (HIP events on the engine's
-stream).
+stream). roofline.copy_kernel_GBps is the engine's own streaming copy kernel
+(nontemporal 16-byte loads and stores, lq_sv_copy) over the same 16 GiB.
+
+serving (the second half of the metric): energy + adjoint-gradient requests
+through the batched server and through the plain one-request-at-a-time
+path, at a few Poisson arrival rates each; the ratio of the best batched
+throughput whose p99 is within the plain path's p99 to the plain path's
+best throughput that keeps up with its arrivals.
 
 cpu_baseline: the reference lightning.qubit C++ (oracle/_ref/liblq_ref.so,
 built from the reference sources) on the host cores, on the first gates of
@@ -38,8 +45,9 @@ HBM_PEAK = 8.0e12
 
 # HBM bytes per specialised pass launch measured with rocprofv3 (FETCH_SIZE and
 # WRITE_SIZE in separate runs of this script, each scaled by the same counter
-# over torch.neg of a known 2 GiB in / 2 GiB out): (precision, qubits) -> bytes
-MEASURED_TRAFFIC = {("c128", 30): 32453777030}
+# over the streaming copy kernel of a known size): (precision, qubits) ->
+# (bytes, profile summary holding the runs)
+MEASURED_TRAFFIC = {("c128", 30): (32453777030, "profiles/lqpass_hea30_c128_traffic.md")}
 
 
 def gate_bytes(op, n, elem):
@@ -71,31 +79,43 @@ def cpu_baseline(n, ops, budget_s, elem):
     dt = time.perf_counter() - t0
     del r
     return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s effective (per-gate byte model)",
-            "cores": threads, "kind": "reference",
+            "cores": threads, "kind": "reference", "host": host_info(),
             "sample": f"first {done} of {len(ops)} gates of the same {n}-qubit circuit, "
                       f"{'complex128' if elem == 16 else 'complex64'}, {dt:.1f} s, OpenMP threads={threads}",
             "gates_per_s": round(done / dt, 4)}
 
 
-def copy_rate(nbytes, reps=5):
-    """HBM rate of a plain streaming kernel (torch.neg: reads and writes
-    nbytes each) on torch's current stream, in bytes/s."""
-    import torch
-
-    src = torch.ones(nbytes // 16, dtype=torch.complex128, device="cuda")
-    dst = torch.empty_like(src)
-    torch.neg(src, out=dst)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
+def copy_rate(sv, reps=5):
+    """HBM rate of the engine's streaming copy kernel (lq_sv_copy: reads and
+    writes the whole state) between two states of sv's size, in bytes/s."""
+    other = type(sv)(sv.num_qubits)
+    other.copy_from(sv)
+    other.synchronize()
+    t0 = time.perf_counter()
     for _ in range(reps):
-        torch.neg(src, out=dst)
-    e1.record()
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del src, dst
-    torch.cuda.empty_cache()
-    return 2 * nbytes / (ms * 1e-3)
+        other.copy_from(sv)
+    other.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    nbytes = len(sv) * (16 if type(sv).__name__.endswith("128") else 8)
+    del other
+    return 2 * nbytes / dt
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "cpu_model": model}
 
 
 def cpu_adjoint_baseline(n, layers, terms):
@@ -189,28 +209,23 @@ def run_adjoint(args):
     print(json.dumps(out))
 
 
-def run_serving(args):
-    """Serving workload: requests with n in [10, 28] (P(n) ~ 2^-((n-10)/2)),
-    depth 4..16 hardware-efficient layers in adjoint form, 50-term Pauli
-    Hamiltonian; each asks for the energy and its full adjoint gradient
-    (complex128). Poisson arrivals at each rate of a sweep, through the
-    batched server and through the plain one-request-at-a-time path."""
+def serving_sweep(count, qps_batched, qps_plain, bits=128, seed=0):
+    """Replay `count` requests at each arrival rate through the batched
+    server, then through the plain path; returns the summary object."""
     from lightning_amd import serving as S
 
-    reqs = S.serving_requests(args.requests, seed=0)
+    reqs = S.serving_requests(count, seed=seed, bits=bits)
     t0 = time.perf_counter()
     for r in reqs:
         r.handles()
     setup_s = time.perf_counter() - t0
     warm = [r for r in reqs if r.n <= 14][:64]
     rows = []
-    qps_plain = [float(x) for x in (args.qps.split(",") if args.qps else ["40", "60", "80", "100", "120", "140"])]
-    qps_batched = [float(x) for x in (args.qps.split(",") if args.qps else ["80", "120", "160", "200", "240", "300"])]
     # the batched sweep runs first: specialised kernels compiled in the
     # background during a sweep stay cached, so any such gain goes to the
     # plain path
     for batched, sweep in ((True, qps_batched), (False, qps_plain)):
-        srv = S.Server(bits=128, batched=batched)
+        srv = S.Server(bits=bits, batched=batched)
         srv.replay(warm, np.zeros(len(warm)))
         for q in sweep:
             res = srv.replay(reqs, S.poisson_arrivals(len(reqs), q, seed=7))
@@ -219,6 +234,7 @@ def run_serving(args):
             rows.append(summ)
             print(json.dumps({"serving_point": summ}), file=sys.stderr, flush=True)
         srv.close()
+
     # a point keeps up with its arrivals when it completes at >= 90% of the
     # offered rate; the plain reference point is its fastest such point
     def keeps_up(r):
@@ -230,16 +246,30 @@ def run_serving(args):
     ok = [r for r in rows if r["mode"] == "batched" and keeps_up(r) and r["p99_ms"] <= best_plain["p99_ms"]]
     best_b = max(ok, key=lambda r: r["req_per_s"]) if ok else None
     ratio = best_b["req_per_s"] / best_plain["req_per_s"] if best_b else None
+    return {"ratio_at_equal_p99": round(ratio, 3) if ratio else None, "requests": count,
+            "dtype": "complex128" if bits == 128 else "complex64",
+            "offered_qps": {"batched": list(qps_batched), "plain": list(qps_plain)},
+            "plain_best": best_plain, "batched_at_equal_p99": best_b, "sweep": rows, "setup_s": round(setup_s, 2)}
+
+
+def run_serving(args):
+    """Serving workload: requests with n in [10, 28] (P(n) ~ 2^-((n-10)/2)),
+    depth 4..16 hardware-efficient layers in adjoint form, 50-term Pauli
+    Hamiltonian; each asks for the energy and its full adjoint gradient
+    (complex128). Poisson arrivals at each rate of a sweep, through the
+    batched server and through the plain one-request-at-a-time path."""
+    qps_plain = [float(x) for x in (args.qps.split(",") if args.qps else ["40", "60", "80", "100", "120", "140"])]
+    qps_batched = [float(x) for x in (args.qps.split(",") if args.qps else ["80", "120", "160", "200", "240", "300"])]
+    sv = serving_sweep(args.requests, qps_batched, qps_plain)
     out = {
         "metric": "adjoint-grad req/s at equal p99, batched vs plain GPU path",
-        "value": round(ratio, 3) if ratio else None, "unit": "x plain throughput (both keeping up with arrivals, batched p99 <= plain p99)",
-        "n_gpus": 1, "steps": len(rows), "warmup": len(warm), "ms_per_step": None, "higher_is_better": True,
+        "value": sv["ratio_at_equal_p99"], "unit": "x plain throughput (both keeping up with arrivals, batched p99 <= plain p99)",
+        "n_gpus": 1, "steps": len(sv["sweep"]), "warmup": 64, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
         "config": {"workload": f"{args.requests} requests, n 10..28 with P(n) ~ 2^-((n-10)/2), depth 4..16 HEA "
                                f"(RZ RY RZ + CZ), 50-term Pauli Hamiltonian, energy + full adjoint gradient",
                    "requests": args.requests, "jit": os.environ.get("LQ_JIT", "1")},
-        "plain_best": best_plain, "batched_at_equal_p99": best_b, "sweep": rows,
-        "setup_s": round(setup_s, 2),
+        "serving": sv,
     }
     print(json.dumps(out))
 
@@ -392,6 +422,10 @@ def main():
                          "brickwork + 3-controlled X with complex64 fidelity; precision: complex64 error per "
                          "workload against the complex128 run")
     ap.add_argument("--requests", type=int, default=1024)
+    ap.add_argument("--no-serving", action="store_true", help="default workload: skip the serving measurement")
+    ap.add_argument("--serving-requests", type=int, default=512)
+    ap.add_argument("--serving-qps-batched", default="150,300,450")
+    ap.add_argument("--serving-qps-plain", default="60,100,140")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
     args = ap.parse_args()
 
@@ -404,9 +438,12 @@ def main():
 
     torch.cuda.set_device(0)
     if args.copy_only:
-        r = copy_rate(4 << 30)
-        print(json.dumps({"kernel": "torch.neg complex128", "bytes_read": 4 << 30, "bytes_written": 4 << 30,
-                          "GBps": round(r / 1e9, 1)}))
+        from lightning_amd import lightning_qubit_ops as L
+        sv = L.StateVectorC128(28)
+        sv.synchronize()
+        r = copy_rate(sv)
+        print(json.dumps({"kernel": "stream_copy_kernel (lq_sv_copy) complex128", "bytes_read": 4 << 30,
+                          "bytes_written": 4 << 30, "GBps": round(r / 1e9, 1)}))
         return
     if args.workload == "adjoint24":
         return run_adjoint(args)
@@ -470,10 +507,14 @@ def main():
     # whole job: the passes' algorithmic bytes over the wall time of the
     # timed steps (fill, reductions and launch gaps included)
     value = 100.0 * (st["bytes"] / args.steps) / (ms_step * 1e-3) / HBM_PEAK if ms_step > 0 else 0.0
-    unfused = sum(gate_bytes(op, n, elem) for op in ops)
-
-    copy_bps = copy_rate((1 << n) * elem)
+    copy_bps = copy_rate(sv)
     cpu = None if args.no_cpu else cpu_baseline(n, ops, args.cpu_seconds, elem)
+    traffic = MEASURED_TRAFFIC.get((args.precision, n))
+    serving = None
+    if not args.no_serving:
+        del sv, M
+        serving = serving_sweep(args.serving_requests, [float(x) for x in args.serving_qps_batched.split(",")],
+                                [float(x) for x in args.serving_qps_plain.split(",")])
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -493,16 +534,17 @@ def main():
                    "n_qubits": n, "layers": args.layers, "gates": len(ops), "seed": 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": MEASURED_TRAFFIC.get((args.precision, n)), "traffic_unit": "bytes per launch",
+                     "traffic": traffic[0] if traffic else None, "traffic_unit": "bytes per launch",
+                     "traffic_source": traffic[1] if traffic else None,
                      "kernel": kernel, "launches_per_step": passes_per_step, "avg_launch_ms": round(pass_ms, 4),
                      "bytes_per_launch": bytes_per_pass,
                      "specialised_passes": pc["specialised"], "generic_passes": pc["generic"],
                      "copy_kernel_GBps": round(copy_bps / 1e9, 1),
                      "frac_of_copy": round(achieved / copy_bps, 4) if copy_bps else None},
         "gates_per_s": round(len(ops) / (ms_step * 1e-3), 1),
-        "effective_unfused_GBps": round(unfused / (ms_step * 1e-3) / 1e9, 1),
         "expvals": {"Z0Z1": ev[0], f"X{n-1}": ev[1]},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out))
 

@@ -72,6 +72,15 @@ int lq_sv_device_ptr(lq_sv_t sv, void **out);
  * nc = 0 for the plain form. "Identity" is a no-op. */
 int lq_sv_apply_op(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
                    const uint8_t *ctrl_vals, uint32_t nc, int inverse, const double *params, uint32_t np);
+
+/* applyOperations(ops, wires, inverses, params) (core/simulators/base/StateVectorBase.hpp:127-172):
+ * nops named gates in order, one call. names: the names back to back, each
+ * NUL-terminated; op i acts on wires[wire_off[i] .. wire_off[i+1]) with
+ * params[param_off[i] .. param_off[i+1]) (param_off and params may be null
+ * for parameter-free ops). */
+int lq_sv_apply_operations(lq_sv_t sv, uint32_t nops, const char *names, const uint32_t *wire_off,
+                           const uint32_t *wires, const uint8_t *inverses, const uint32_t *param_off,
+                           const double *params);
 /* applyMatrix / applyControlledMatrix (lq/StateVectorLQubit.hpp:571-710;
  * binding core/bindings/Bindings.hpp:152-200); at most 10 target wires */
 int lq_sv_apply_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,

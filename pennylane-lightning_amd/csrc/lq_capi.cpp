@@ -122,6 +122,27 @@ int lq_sv_apply_op(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t
     });
 }
 
+int lq_sv_apply_operations(lq_sv_t sv, uint32_t nops, const char *names, const uint32_t *wire_off,
+                           const uint32_t *wires, const uint8_t *inverses, const uint32_t *param_off,
+                           const double *params) {
+    return guard([&] {
+        LQ_ABORT_IF(nops && (!names || !wire_off || !inverses), "null argument");
+        StateVector &s = S(sv);
+        const char *nm = names;
+        std::string name;
+        for (uint32_t i = 0; i < nops; ++i) {
+            const size_t len = std::strlen(nm);
+            name.assign(nm, len);
+            nm += len + 1;
+            LQ_ABORT_IF(wire_off[i + 1] < wire_off[i], "invalid wire offsets");
+            const uint32_t np = param_off ? param_off[i + 1] - param_off[i] : 0;
+            LQ_ABORT_IF(np && !params, "null parameters");
+            s.apply_named(name, wires + wire_off[i], static_cast<int>(wire_off[i + 1] - wire_off[i]), nullptr, nullptr,
+                          0, inverses[i] != 0, np ? params + param_off[i] : nullptr, static_cast<int>(np));
+        }
+    });
+}
+
 int lq_sv_apply_matrix(lq_sv_t sv, const void *matrix, const uint32_t *wires, uint32_t nw, const uint32_t *ctrls,
                        const uint8_t *ctrl_vals, uint32_t nc, int inverse) {
     return guard([&] {
@@ -580,8 +601,8 @@ int lq_sv_pass_counts(lq_sv_t sv, int reset, uint64_t *specialised, uint64_t *ge
 int lq_set_split_fallback(int mode, int *previous) {
     return guard([&] {
         LQ_ABORT_IF(mode < 0 || mode > 2, "split fallback mode must be 0, 1 or 2");
-        if (previous) *previous = tuning().split_fallback;
-        tuning().split_fallback = mode;
+        const int old = tuning().split_fallback.exchange(mode);
+        if (previous) *previous = old;
     });
 }
 

@@ -54,8 +54,9 @@ void HostPinned::ensure(size_t b) {
 }
 
 EngineTuning &tuning() {
-    static EngineTuning t = [] {
-        EngineTuning e;
+    static EngineTuning &t = *[] {
+        auto *p = new EngineTuning; // lives until exit
+        EngineTuning &e = *p;
         if (const char *s = std::getenv("LQ_TILE_BITS")) e.kt_max = std::atoi(s);
         if (const char *s = std::getenv("LQ_LOW_BITS")) e.low_bits = std::atoi(s);
         if (const char *s = std::getenv("LQ_PASS_BUDGET")) e.budget = std::atof(s);
@@ -64,8 +65,8 @@ EngineTuning &tuning() {
         if (const char *s = std::getenv("LQ_PLAN_DEBUG")) e.plan_debug = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_PLAN_CACHE")) e.plan_cache = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_PLAN_LOOKAHEAD")) e.lookahead = std::atoi(s) != 0;
-        if (const char *s = std::getenv("LQ_SPLIT_FALLBACK")) e.split_fallback = std::atoi(s);
-        return e;
+        if (const char *s = std::getenv("LQ_SPLIT_FALLBACK")) e.split_fallback.store(std::atoi(s));
+        return p;
     }();
     return t;
 }
@@ -218,9 +219,11 @@ StateVector::StateVector(int n, int dtype) : n_(n), dtype_(dtype ? 1 : 0) {
     LQ_ABORT_IF(n < 1 || n > 40, "number of qubits must be in [1, 40]");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&staged_, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&copied_, hipEventDisableTiming), "hipEventCreate");
     hipError_t e = hipMalloc(&d_, bytes());
     if (e != hipSuccess) {
         d_ = nullptr;
+        (void)hipEventDestroy(copied_);
         (void)hipEventDestroy(staged_);
         (void)hipStreamDestroy(stream_);
         hip_check(e, "hipMalloc(state vector)");
@@ -238,6 +241,7 @@ StateVector::~StateVector() {
     ws_.clear();
     if (d_) (void)hipFree(d_);
     if (staged_) (void)hipEventDestroy(staged_);
+    if (copied_) (void)hipEventDestroy(copied_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -324,14 +328,17 @@ void StateVector::set_basis_index(uint64_t index) {
     queue_.clear();
     // deferred: the next program's first specialised pass forms the state,
     // any other use writes it first
-    pending_basis_ = static_cast<int64_t>(index);
+    pend_.assign(2 * static_cast<size_t>(n_), cd(0.0, 0.0));
+    for (int q = 0; q < n_; ++q) pend_[2 * static_cast<size_t>(q) + ((index >> q) & 1ull)] = 1.0;
+    prep_pending_ = true;
 }
 
 void StateVector::materialize() {
-    if (pending_basis_ < 0) return;
-    const uint64_t index = static_cast<uint64_t>(pending_basis_);
-    pending_basis_ = -1;
-    dispatch(dtype_, [&](auto t) { launch_fill_basis<decltype(t)>(d_, n_, index, stream_); });
+    if (!prep_pending_) return;
+    prep_pending_ = false;
+    ProductFactors f{};
+    for (size_t i = 0; i < pend_.size(); ++i) f.f[i] = make_double2(pend_[i].real(), pend_[i].imag());
+    dispatch(dtype_, [&](auto t) { launch_fill_product<decltype(t)>(d_, n_, f, stream_); });
 }
 
 void StateVector::set_basis_state(const uint32_t *bits, const uint32_t *wires, int nw) {
@@ -358,7 +365,7 @@ void StateVector::set_state_vector(const void *host, const uint32_t *wires, int 
 void StateVector::update_data(const void *host, uint64_t len) {
     LQ_ABORT_IF(len != length(), "The size of provided data must be equal to the statevector size.");
     queue_.clear();
-    pending_basis_ = -1; // every amplitude is overwritten
+    prep_pending_ = false; // every amplitude is overwritten
     hip_check(hipMemcpyAsync(d_, host, bytes(), hipMemcpyHostToDevice, stream_), "hipMemcpyAsync");
     synchronize();
 }
@@ -372,14 +379,22 @@ void StateVector::get_data(void *host, uint64_t len) {
 
 void StateVector::copy_from(StateVector &src) {
     LQ_ABORT_IF(src.n_ != n_ || src.dtype_ != dtype_, "state vectors differ in size or precision");
+    if (&src == this) {
+        flush();
+        return;
+    }
     src.flush();
     // data() may enqueue the write of a deferred reset on src's stream: take
     // it before waiting for that stream
     const void *sp = src.data();
     src.synchronize();
     queue_.clear();
-    pending_basis_ = -1; // every amplitude is overwritten
-    hip_check(hipMemcpyAsync(d_, sp, bytes(), hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync");
+    prep_pending_ = false; // every amplitude is overwritten
+    launch_stream_copy(sp, d_, bytes(), stream_);
+    // src's next write (a gate program, updateData, a deferred reset) runs on
+    // src's stream: it waits until this copy has read the amplitudes
+    hip_check(hipEventRecord(copied_, stream_), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(src.stream_, copied_, 0), "hipStreamWaitEvent");
 }
 
 void StateVector::enqueue(LOp op) {
@@ -402,6 +417,15 @@ void StateVector::synchronize() {
 
 std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector<void *> &states, int nslots) {
     if (prog.empty()) return std::vector<cd>(static_cast<size_t>(nslots), 0.0);
+    if (prep_pending_ && states.size() == 1 && states[0] == d_) {
+        // the state is a product state not yet written: its leading
+        // one-qubit gates become factors of that product
+        std::vector<LOp> rest = prog;
+        if (absorb_into_product(rest, pend_) > 0) {
+            if (rest.empty()) return std::vector<cd>(static_cast<size_t>(nslots), 0.0);
+            return execute(plan(rest, 1), states, nslots);
+        }
+    }
     return execute(plan(prog, static_cast<int>(states.size())), states, nslots);
 }
 
@@ -502,7 +526,8 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
         split_1q_rotations(split);
         passes = plan_cached(split, scfg);
         const size_t missing = lookup_all(fns);
-        if (missing && t.split_fallback != 2) {
+        const int fallback = t.split_fallback.load();
+        if (missing && fallback != 2) {
             // The generic kernel interprets the op list, so its pass time grows
             // with the ops: measured on the 30-qubit HEA passes, about 0.28 of
             // a specialised pass per op of the split plan (phase tables, real
@@ -513,7 +538,7 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
             for (size_t p = 0; p < passes.size(); ++p)
                 split_cost += (fns[p] || !passes[p].reg) ? 1.0 : 0.28 * static_cast<double>(passes[p].ops.size());
             for (const PassPlan &pp : plain) plain_cost += std::max(1.0, 0.16 * static_cast<double>(pp.ops.size()));
-            if (t.split_fallback == 0 || plain_cost < split_cost) {
+            if (fallback == 0 || plain_cost < split_cost) {
                 passes.swap(plain);
                 fns.assign(passes.size(), nullptr);
             }
@@ -562,7 +587,9 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         st_off[p] = st_bytes;
         st_bytes += passes[p].stages.size() * sizeof(TileStage);
         coef_off[p] = coef_n;
-        coef_n += passes[p].coef.size() + 1; // + the deferred-reset entry read by specialised passes
+        // + the deferred-preparation marker and the product factors read by
+        // specialised one-state passes
+        coef_n += passes[p].coef.size() + 1 + 2 * static_cast<size_t>(n_);
         part_off[p] = part_n;
         const uint64_t ntiles = 1ull << (n_ - static_cast<int>(passes[p].tbits.size()));
         for (size_t s = 0; s < passes[p].slot_ids.size(); ++s) {
@@ -582,12 +609,12 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
     // a specialised one-state pass reading and writing it (a read-only pass
     // would leave the memory unwritten); otherwise it is written now.
     bool form_basis = false;
-    if (pending_basis_ >= 0) {
+    if (prep_pending_) {
         bool reads = false;
         for (void *sp : states) reads |= sp == d_;
         if (reads) {
-            form_basis = ns == 1 && fns[0] != nullptr && passes[0].reg && (passes[0].load_mask & 1u) &&
-                         (passes[0].store_mask & 1u) && !jit_persistent(dtype_);
+            form_basis = ns == 1 && states[0] == d_ && fns[0] != nullptr && passes[0].reg &&
+                         (passes[0].load_mask & 1u) && (passes[0].store_mask & 1u);
             if (!form_basis) materialize();
         }
     }
@@ -603,10 +630,12 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         double2 *c = reinterpret_cast<double2 *>(hs + coef_at) + coef_off[p];
         for (size_t i = 0; i < passes[p].coef.size(); ++i)
             c[i] = make_double2(passes[p].coef[i].real(), passes[p].coef[i].imag());
-        c[passes[p].coef.size()] = (p == 0 && form_basis) ? make_double2(1.0, static_cast<double>(pending_basis_))
-                                                          : make_double2(-1.0, 0.0);
+        const bool form = p == 0 && form_basis;
+        c[passes[p].coef.size()] = make_double2(form ? 1.0 : -1.0, 0.0);
+        for (size_t i = 0; i < 2 * static_cast<size_t>(n_); ++i)
+            c[passes[p].coef.size() + 1 + i] = form ? make_double2(pend_[i].real(), pend_[i].imag()) : make_double2(0.0, 0.0);
     }
-    if (form_basis) pending_basis_ = -1;
+    if (form_basis) prep_pending_ = false;
     if (!ranges.empty()) std::memcpy(hs + ranges_at, ranges.data(), ranges.size() * sizeof(SlotRange));
     prog_.ensure(total);
     hip_check(hipMemcpyAsync(prog_.p, hs, total, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");

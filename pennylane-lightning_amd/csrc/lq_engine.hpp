@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <memory>
 #include <string>
 #include <vector>
@@ -98,7 +99,8 @@ struct EngineTuning {
     // while some specialised kernels of a split plan are missing: 0 run the
     // unsplit plan, 1 choose by the op-count estimate, 2 run the split plan
     // (its missing passes in the generic kernel)
-    int split_fallback = 1;
+    // (written through lq_set_split_fallback while other threads plan)
+    std::atomic<int> split_fallback{1};
 };
 EngineTuning &tuning();
 
@@ -209,16 +211,20 @@ class StateVector {
   private:
     int n_, dtype_;
     void *d_ = nullptr;
-    // reset deferred: the state is the basis state |pending_basis_> but not yet
+    // preparation deferred: the state is the product state with amplitude
+    // prod_q pend_[2q + bit q of the index] (a basis state after a reset, with
+    // the leading one-qubit gates of the next program folded in) but not yet
     // written; the first specialised pass of the next program forms it, any
     // other use writes it first (materialize)
-    int64_t pending_basis_ = -1;
+    bool prep_pending_ = false;
+    std::vector<cd> pend_;
     void materialize();
     hipStream_t stream_ = nullptr;
     std::vector<LOp> queue_;
     DevBuf prog_, partials_, slots_, small_;
     HostPinned staging_;
     hipEvent_t staged_ = nullptr;
+    hipEvent_t copied_ = nullptr; // a copy out of another state finished reading it
     std::vector<std::unique_ptr<DevBuf>> ws_;
     bool prof_ = false;
     struct Pending {

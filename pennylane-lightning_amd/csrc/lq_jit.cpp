@@ -40,6 +40,9 @@ __device__ __forceinline__ cplx cm2(const cplx a, const cplx b, const cplx c, co
     return mk(a.x * b.x - a.y * b.y + c.x * d.x - c.y * d.y, a.x * b.y + a.y * b.x + c.x * d.y + c.y * d.x);
 }
 __device__ __forceinline__ cplx add(const cplx a, const cplx b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zm(const double2 a, const double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
 __device__ __forceinline__ cplx ng(const cplx a) { return mk(-a.x, -a.y); }
 __device__ __forceinline__ void d1(cplx &a, cplx &b, const cplx m0, const cplx m1, const cplx m2, const cplx m3) {
     const cplx x = a, y = b;
@@ -159,7 +162,7 @@ struct Mapping {
 class Gen {
   public:
     Gen(const PassPlan &p, int n, int dtype, int ns)
-        : p_(p), dtype_(dtype), ns_(ns), nt_(jit_config().nontemporal), persist_(jit_persistent(dtype)) {
+        : p_(p), dtype_(dtype), ns_(ns), nt_(jit_config().nontemporal) {
         kt_ = static_cast<int>(p.tbits.size());
         rb_ = kt_ - 8;
         E_ = 1 << rb_;
@@ -191,74 +194,41 @@ class Gen {
         for (int i = 0; i < rb_; ++i) m0.r[i] = 8 + i;
         for (int j = 0; j < 8; ++j) m0.h[j] = j;
         for (int s = 0; s < ns_; ++s) o << "  cplx v" << s << "[" << E_ << "];\n";
-        if (!persist_) {
-            // workgroups are dealt round-robin to the 8 XCDs: let XCD x walk
-            // the x-th eighth of the tiles in order, so neighbouring segments
-            // are streamed by one XCD at a time (ntiles is a power of two)
-            o << "  const unsigned long long bid = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * gridDim.x;\n";
-            o << "  if (bid >= ntiles) return;\n";
-            o << "  const unsigned long long tile = ntiles >= 8ull ? (bid & 7ull) * (ntiles >> 3) + (bid >> 3) : bid;\n";
-            // load straight into the first stage's mapping when its thread
-            // bits start with the contiguous low bits (the loads coalesce as
-            // in the default mapping): no entry transpose
-            Mapping lm = m0;
-            const TileStage &s0 = p_.stages.front();
-            if (s0.kind == STAGE_REG && s0.entry == ENTRY_TRANSPOSE) {
-                Mapping m{};
-                for (int i = 0; i < rb_; ++i) m.r[i] = s0.rbits[i];
-                for (int j = 0; j < 8; ++j) m.h[j] = s0.hbits[j];
-                if (coalesces(m)) lm = m;
-            }
-            o << "  {\n";
-            emit_base_gl("tile", "gl0", lm);
-            if (ns_ == 1) {
-                // the entry after the pass's coefficients: (1, k) when the state
-                // is the basis state |k> not yet written (a deferred reset); the
-                // pass then forms its input instead of reading it
-                o << "  const double2 bmark = ((const kcoef *)coef)[" << p_.coef.size() << "];\n";
-                o << "  if (bmark.x > 0.0) {\n";
-                o << "    const unsigned long long bidx = (unsigned long long)bmark.y;\n";
-                for (int e = 0; e < E_; ++e)
-                    o << "    v0[" << e << "] = mk((gl0_ | " << hex64(elem_global(lm, e)) << ") == bidx ? (real)1 : (real)0, (real)0);\n";
-                o << "  } else {\n";
-                emit_loads("v", "gl0", lm);
-                o << "  }\n";
-            } else {
-                emit_loads("v", "gl0", lm);
-            }
-            o << "  }\n";
-            emit_tile_body(m0, nslots, lm);
-        } else {
-            // persistent workgroups: each walks tiles blockIdx.x + k * gridDim.x
-            // and loads the next tile while it works on the current one
-            for (int s = 0; s < ns_; ++s) o << "  cplx n" << s << "[" << E_ << "];\n";
-            o << "  unsigned long long tile = blockIdx.x;\n";
-            o << "  if (tile >= ntiles) return;\n";
-            o << "  {\n";
-            emit_base_gl("tile", "gl0", m0);
-            emit_loads("v", "gl0", m0);
-            o << "  }\n";
-            o << "  while (true) {\n";
-            o << "  const unsigned long long ntile = tile + gridDim.x;\n";
-            o << "  const bool more = ntile < ntiles;\n";
-            // unconditional (the last tile loads itself again), so the loads
-            // stay in flight across the work on the current tile
-            o << "  const unsigned long long ltile = more ? ntile : tile;\n";
-            o << "  {\n";
-            emit_base_gl("ltile", "gln", m0);
-            emit_loads("n", "gln", m0);
-            o << "  }\n";
-            emit_tile_body(m0, nslots, m0);
-            o << "  if (!more) break;\n";
-            o << "  tile = ntile;\n";
-            for (int s = 0; s < ns_; ++s) {
-                if (!((p_.load_mask >> s) & 1u)) continue;
-                for (int e = 0; e < E_; ++e) o << "  v" << s << "[" << e << "] = n" << s << "[" << e << "];\n";
-            }
-            o << "  __syncthreads();\n";
-            lds_used_ = false;
-            o << "  }\n";
+        // workgroups are dealt round-robin to the 8 XCDs: let XCD x walk
+        // the x-th eighth of the tiles in order, so neighbouring segments
+        // are streamed by one XCD at a time (ntiles is a power of two)
+        o << "  const unsigned long long bid = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * gridDim.x;\n";
+        o << "  if (bid >= ntiles) return;\n";
+        o << "  const unsigned long long tile = ntiles >= 8ull ? (bid & 7ull) * (ntiles >> 3) + (bid >> 3) : bid;\n";
+        // load straight into the first stage's mapping when its thread
+        // bits start with the contiguous low bits (the loads coalesce as
+        // in the default mapping): no entry transpose
+        Mapping lm = m0;
+        const TileStage &s0 = p_.stages.front();
+        if (s0.kind == STAGE_REG && s0.entry == ENTRY_TRANSPOSE) {
+            Mapping m{};
+            for (int i = 0; i < rb_; ++i) m.r[i] = s0.rbits[i];
+            for (int j = 0; j < 8; ++j) m.h[j] = s0.hbits[j];
+            if (coalesces(m)) lm = m;
         }
+        o << "  {\n";
+        emit_base_gl("tile", "gl0", lm);
+        if (ns_ == 1) {
+            // the entry after the pass's coefficients is (1, 0) when the state
+            // is a product state not yet written (a deferred reset, with the
+            // leading one-qubit gates folded in); the pass then forms its
+            // input from the factors that follow instead of reading it
+            o << "  const double2 bmark = ((const kcoef *)coef)[" << p_.coef.size() << "];\n";
+            o << "  if (bmark.x > 0.0) {\n";
+            emit_product(lm, p_.coef.size() + 1);
+            o << "  } else {\n";
+            emit_loads("v", "gl0", lm);
+            o << "  }\n";
+        } else {
+            emit_loads("v", "gl0", lm);
+        }
+        o << "  }\n";
+        emit_tile_body(m0, nslots, lm);
         o << "}\n";
         return o.str();
     }
@@ -267,7 +237,6 @@ class Gen {
     const PassPlan &p_;
     int dtype_, ns_;
     bool nt_;
-    bool persist_;
 
     void emit_base_gl(const char *tile, const char *gl, const Mapping &m) {
         o_ << "  const unsigned long long " << (std::string(gl) == "gl0" ? "b0" : "bn") << " = "
@@ -276,6 +245,37 @@ class Gen {
         for (int j = 0; j < 8; ++j) pos[j] = geom_.tb[m.h[j]];
         o_ << "  const unsigned long long " << gl << "_ = " << (std::string(gl) == "gl0" ? "b0" : "bn") << " | "
            << deposit("tid", pos, 8, true) << ";\n";
+    }
+    // v0 = the product state prod_q f_q[bit q] over the tile, f_q[b] at
+    // coefficient F + 2q + b: the factors of the bits fixed per thread
+    // (outside and thread bits) multiply into one per-thread value, the
+    // register bits' factors into a 2^rb table (FP64 in both precisions)
+    void emit_product(const Mapping &m, size_t F) {
+        std::ostringstream &o = o_;
+        int rg[kMaxRegBits];
+        uint64_t regs = 0;
+        for (int i = 0; i < rb_; ++i) {
+            rg[i] = geom_.tb[m.r[i]];
+            regs |= 1ull << rg[i];
+        }
+        auto fac = [&](int q, int b) {
+            return "((const kcoef *)coef)[" + std::to_string(F + 2 * static_cast<size_t>(q) + static_cast<size_t>(b)) + "]";
+        };
+        o << "    double2 pw[" << E_ << "];\n";
+        o << "    pw[0] = make_double2(1.0, 0.0);\n";
+        for (int q = 0; q < geom_.n; ++q) {
+            if ((regs >> q) & 1ull) continue;
+            o << "    pw[0] = zm(pw[0], ((gl0_ >> " << q << ") & 1ull) ? " << fac(q, 1) << " : " << fac(q, 0) << ");\n";
+        }
+        for (int i = 0; i < rb_; ++i) {
+            o << "    { const double2 f0 = " << fac(rg[i], 0) << ", f1 = " << fac(rg[i], 1) << ";\n";
+            for (int e = (1 << i) - 1; e >= 0; --e) {
+                o << "      pw[" << (e | (1 << i)) << "] = zm(pw[" << e << "], f1);\n";
+                o << "      pw[" << e << "] = zm(pw[" << e << "], f0);\n";
+            }
+            o << "    }\n";
+        }
+        for (int e = 0; e < E_; ++e) o << "    v0[" << e << "] = mk((real)pw[" << e << "].x, (real)pw[" << e << "].y);\n";
     }
     void emit_loads(const char *dst, const char *gl, const Mapping &m) {
         for (int s = 0; s < ns_; ++s) {
@@ -341,7 +341,7 @@ class Gen {
         if (p_.store_mask) {
             // store straight from the last stage's mapping when it coalesces
             const char *gs = "gl0";
-            if (p_.end_transpose && !persist_ && coalesces(cur_)) {
+            if (p_.end_transpose && coalesces(cur_)) {
                 emit_gl("gls", cur_);
                 gs = "gls";
             } else {
@@ -822,13 +822,6 @@ std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns) {
 // ---------------------------------------------------------------------------
 // Compile service
 // ---------------------------------------------------------------------------
-bool jit_persistent(int dtype) {
-    // complex64 kernels stay one workgroup per tile: a persistent complex64
-    // pass with dot products stopped making progress on the GPU once, cause
-    // not found, so that form is off until it is
-    return jit_config().persistent && dtype == 1;
-}
-
 JitConfig &jit_config() {
     static JitConfig c = [] {
         JitConfig j;
@@ -837,7 +830,6 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_MIN_QUBITS")) j.min_qubits = std::atoi(s);
         if (const char *s = std::getenv("LQ_JIT_THREADS")) j.threads = std::max(1, std::atoi(s));
         if (const char *s = std::getenv("LQ_JIT_NT")) j.nontemporal = std::atoi(s) != 0;
-        if (const char *s = std::getenv("LQ_JIT_PERSIST")) j.persistent = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_SPLIT")) j.split_rotations = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_LDS32")) j.lds32 = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_SCALE")) j.scale_rotations = std::atoi(s) != 0;
@@ -1064,9 +1056,11 @@ hipFunction_t jit_lookup_plan(const PassPlan &p, int n, int dtype, int ns) {
     // every field the generated source depends on (ops and stages are
     // zero-initialised PODs, so their bytes are a faithful key)
     std::string key;
-    const int32_t hdr[9] = {n, dtype, ns, p.reg ? 1 : 0, static_cast<int32_t>(p.tbits.size()),
-                            static_cast<int32_t>(p.load_mask), static_cast<int32_t>(p.store_mask),
-                            static_cast<int32_t>(p.end_transpose), static_cast<int32_t>(p.slot_ids.size())};
+    // (the coefficient count places the deferred-preparation marker)
+    const int32_t hdr[10] = {n, dtype, ns, p.reg ? 1 : 0, static_cast<int32_t>(p.tbits.size()),
+                             static_cast<int32_t>(p.load_mask), static_cast<int32_t>(p.store_mask),
+                             static_cast<int32_t>(p.end_transpose), static_cast<int32_t>(p.slot_ids.size()),
+                             static_cast<int32_t>(p.coef.size())};
     key.append(reinterpret_cast<const char *>(hdr), sizeof(hdr));
     key.append(reinterpret_cast<const char *>(p.tbits.data()), p.tbits.size() * sizeof(int));
     key.append(reinterpret_cast<const char *>(p.ops.data()), p.ops.size() * sizeof(TileOp));
@@ -1083,35 +1077,8 @@ void jit_launch(hipFunction_t f, int dtype, void *sv0, void *sv1, const void *co
     const void *a2 = coef;
     unsigned long long nt = ntiles;
     void *args[] = {&a0, &a1, &a2, &a3, &nt};
-    uint32_t gx, gy = 1;
-    if (jit_persistent(dtype)) {
-        // one resident wave of workgroups; each loops over its tiles
-        static std::mutex mu;
-        static std::unordered_map<hipFunction_t, uint32_t> resident;
-        uint32_t slots = 0;
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            auto it = resident.find(f);
-            if (it != resident.end()) slots = it->second;
-        }
-        if (!slots) {
-            int per_cu = 0, cus = 0, dev = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-                hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, lds_bytes) != hipSuccess ||
-                per_cu < 1 || cus < 1) {
-                per_cu = 1;
-                cus = 256;
-            }
-            slots = static_cast<uint32_t>(per_cu * cus);
-            std::lock_guard<std::mutex> lk(mu);
-            resident[f] = slots;
-        }
-        gx = static_cast<uint32_t>(ntiles < slots ? ntiles : slots);
-    } else {
-        gx = static_cast<uint32_t>(ntiles < 65536 ? ntiles : 65536);
-        gy = static_cast<uint32_t>((ntiles + gx - 1) / gx);
-    }
+    const uint32_t gx = static_cast<uint32_t>(ntiles < 65536 ? ntiles : 65536);
+    const uint32_t gy = static_cast<uint32_t>((ntiles + gx - 1) / gx);
     const hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, kBlock, 1, 1, static_cast<unsigned>(lds_bytes), s, args, nullptr);
     if (e != hipSuccess) {
         const std::string m = std::string("hipModuleLaunchKernel: ") + hipGetErrorString(e);

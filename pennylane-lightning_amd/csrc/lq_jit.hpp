@@ -26,8 +26,6 @@ struct JitConfig {
     int threads = 8;        // background compile threads
     bool nontemporal = true;  // state loads/stores marked nontemporal (streamed, not retained)
     bool split_rotations = true; // 2x2 gates as phase / real rotation / phase in specialised passes
-    bool persistent = false;  // workgroups loop over tiles, loading the next while computing
-                              // (complex128 only; measured slower than one workgroup per tile)
     bool scale_rotations = true; // specialised passes: rotations as [[1, -t], [t, 1]]
     bool lds32 = true;        // transposes through 32 KiB of LDS (halves, one state at a time)
                               // so that more workgroups fit per CU
@@ -56,8 +54,6 @@ void jit_shutdown();
 // Counters: kernels ready, compiles failed, compiles pending.
 void jit_stats(int *ready, int *failed, int *pending);
 
-// Whether kernels of this precision are generated in the persistent form.
-bool jit_persistent(int dtype);
 // Launch a specialised pass (arguments as the generic register kernel).
 void jit_launch(hipFunction_t f, int dtype, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
                 size_t lds_bytes, hipStream_t s);

@@ -304,11 +304,49 @@ __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const double *parti
     out[col] = s + cmp;
 }
 
+// One workgroup per segment of 2^(lb + mb) amplitudes: the products over the
+// low lb and the next mb bits are tables in LDS, the product over the
+// segment's own bits one value; every amplitude is then two multiplies.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void fill_basis_kernel(typename VecT<T>::type *a, uint64_t N, uint64_t index) {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride)
-        a[i] = mk<T>(i == index ? T(1) : T(0), T(0));
+__global__ __launch_bounds__(kBlock) void fill_product_kernel(typename VecT<T>::type *a, int n, int lb, int mb,
+                                                             ProductFactors f) {
+    __shared__ double2 lo[256], mid[256];
+    __shared__ double2 seg;
+    const uint32_t tid = threadIdx.x;
+    auto prod = [&](uint64_t bits, int from, int count) {
+        double2 p = make_double2(1.0, 0.0);
+        for (int j = 0; j < count; ++j) {
+            const double2 g = f.f[2 * (from + j) + ((bits >> j) & 1ull)];
+            p = make_double2(p.x * g.x - p.y * g.y, p.x * g.y + p.y * g.x);
+        }
+        return p;
+    };
+    if (tid < (1u << lb)) lo[tid] = prod(tid, 0, lb);
+    if (tid < (1u << mb)) mid[tid] = prod(tid, lb, mb);
+    if (tid == 0) seg = prod(blockIdx.x, lb + mb, n - lb - mb);
+    __syncthreads();
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) << (lb + mb);
+    const uint32_t per = 1u << (lb + mb);
+    for (uint32_t i = tid; i < per; i += kBlock) {
+        const double2 m = mid[i >> lb], l = lo[i & ((1u << lb) - 1u)];
+        const double2 h = make_double2(seg.x * m.x - seg.y * m.y, seg.x * m.y + seg.y * m.x);
+        a[base + i] = mk<T>(static_cast<T>(h.x * l.x - h.y * l.y), static_cast<T>(h.x * l.y + h.y * l.x));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void stream_copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                            uint64_t N) {
+    typedef double dvec __attribute__((ext_vector_type(2)));
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock * 4;
+    for (uint64_t i = (static_cast<uint64_t>(blockIdx.x) * kBlock * 4) + threadIdx.x; i < N; i += stride) {
+        dvec v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * kBlock < N) v[u] = __builtin_nontemporal_load(reinterpret_cast<const dvec *>(src + i + u * kBlock));
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * kBlock < N) __builtin_nontemporal_store(v[u], reinterpret_cast<dvec *>(dst + i + u * kBlock));
+    }
 }
 
 template <typename T>
@@ -487,13 +525,24 @@ void launch_reduce_rows(const double *partials, uint32_t nrows, uint32_t ncols, 
                        ncols, out);
 }
 
-template <typename T> void launch_fill_basis(void *a, int n, uint64_t index, hipStream_t s) {
+template <typename T> void launch_fill_product(void *a, int n, const ProductFactors &f, hipStream_t s) {
     using V = typename VecT<T>::type;
-    const uint64_t N = 1ull << n;
-    hipLaunchKernelGGL(fill_basis_kernel<T>, dim3(stream_blocks(N)), dim3(kBlock), 0, s, static_cast<V *>(a), N, index);
+    const int lb = n < 8 ? n : 8;
+    const int mb = (n - lb) < 8 ? (n - lb) : 8;
+    const uint32_t segs = 1u << (n - lb - mb); // n <= 40: at most 2^24 segments
+    hipLaunchKernelGGL(fill_product_kernel<T>, dim3(segs), dim3(kBlock), 0, s, static_cast<V *>(a), n, lb, mb, f);
 }
-template void launch_fill_basis<double>(void *, int, uint64_t, hipStream_t);
-template void launch_fill_basis<float>(void *, int, uint64_t, hipStream_t);
+template void launch_fill_product<double>(void *, int, const ProductFactors &, hipStream_t);
+template void launch_fill_product<float>(void *, int, const ProductFactors &, hipStream_t);
+
+void launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s) {
+    const uint64_t N = bytes / 16;
+    const uint64_t per = static_cast<uint64_t>(kBlock) * 4;
+    const uint64_t b = (N + per - 1) / per;
+    const uint32_t blocks = static_cast<uint32_t>(b < 16384 ? (b ? b : 1) : 16384);
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(kBlock), 0, s, static_cast<const double2 *>(src),
+                       static_cast<double2 *>(dst), N);
+}
 
 template <typename T> void launch_scale(void *a, int n, double re, double im, hipStream_t s) {
     using V = typename VecT<T>::type;

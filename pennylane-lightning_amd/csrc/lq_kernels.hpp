@@ -91,7 +91,13 @@ void launch_probs_gather(const void *a, int n, const int *bits, int nw, double *
 void launch_reduce_rows(const double *partials, uint32_t nrows, uint32_t ncols, double *out,
                         hipStream_t s);
 
-template <typename T> void launch_fill_basis(void *a, int n, uint64_t index, hipStream_t s);
+// a[i] = prod_q f[2q + bit q of i] (FP64 products), q < n <= 40
+struct ProductFactors {
+    double2 f[80];
+};
+template <typename T> void launch_fill_product(void *a, int n, const ProductFactors &f, hipStream_t s);
+// dst = src, bytes a multiple of 16 (nontemporal 16-byte loads and stores)
+void launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
 template <typename T> void launch_scale(void *a, int n, double re, double im, hipStream_t s);
 // zero amplitudes whose bit `bit` != branch, then scale the rest by `factor`
 template <typename T>

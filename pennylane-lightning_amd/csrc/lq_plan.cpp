@@ -255,6 +255,40 @@ std::vector<cd> as_2x2(const LOp &o) {
 }
 } // namespace
 
+size_t absorb_into_product(std::vector<LOp> &prog, std::vector<cd> &f) {
+    const int nq = static_cast<int>(f.size() / 2);
+    LQ_ABORT_IF(nq < 1 || nq > 64, "product state of unsupported size");
+    const uint64_t all = nq == 64 ? ~0ull : ((1ull << nq) - 1ull);
+    uint64_t blocked = 0;
+    size_t folded = 0, i = 0;
+    std::vector<LOp> rest;
+    rest.reserve(prog.size());
+    for (; i < prog.size() && blocked != all; ++i) {
+        LOp &o = prog[i];
+        const bool own = !o.is_dot() && !(o.flags & TF_ZERO_UNCTRL) && o.cmask == 0 && o.smask == 1u;
+        if (own && o.kind == TOP_DIAG && o.k == 0 && o.coef.size() == 1) {
+            f[0] *= o.coef[0];
+            f[1] *= o.coef[0];
+            ++folded;
+            continue;
+        }
+        if (own && single_bit_1q(o) && o.bits[0] < nq && !((blocked >> o.bits[0]) & 1ull)) {
+            const size_t q = static_cast<size_t>(o.bits[0]);
+            const std::vector<cd> m = as_2x2(o);
+            const cd a = f[2 * q], b = f[2 * q + 1];
+            f[2 * q] = m[0] * a + m[1] * b;
+            f[2 * q + 1] = m[2] * a + m[3] * b;
+            ++folded;
+            continue;
+        }
+        blocked |= o.is_dot() ? all : touch_mask(o);
+        rest.push_back(std::move(o));
+    }
+    for (; i < prog.size(); ++i) rest.push_back(std::move(prog[i]));
+    prog.swap(rest);
+    return folded;
+}
+
 void split_1q_rotations(std::vector<LOp> &prog) {
     std::vector<LOp> out;
     out.reserve(prog.size() * 3);

@@ -67,6 +67,14 @@ bool try_merge_1q(std::vector<LOp> &queue, const LOp &op);
 // factors do not reproduce U to 1e-13 are kept whole.
 void split_1q_rotations(std::vector<LOp> &prog);
 
+// The state before `prog` is the product state with amplitude
+// prod_q f[2q + bit q of the index] (f holds one pair per physical bit).
+// Fold into f every uncontrolled one-bit gate of state 0 that comes before
+// any other op touching its bit (such a gate commutes with everything before
+// it), and global phases; remove them from `prog`. Dots read the whole state:
+// nothing after the first one is folded. Returns the number of ops folded.
+size_t absorb_into_product(std::vector<LOp> &prog, std::vector<cd> &f);
+
 struct PassPlan {
     std::vector<int> tbits;        // tile bits, ascending
     std::vector<TileOp> ops;

@@ -131,11 +131,12 @@ void Server::batch_loop() {
     std::vector<std::vector<ReqP>> flight(K);
     auto retire = [&](int i, bool block) {
         if (flight[i].empty()) return false;
-        if (!block && !runners[i]->done()) return false;
         std::vector<double> e;
         std::vector<std::vector<double>> g;
         std::string err;
         try {
+            // a device error seen by the query fails the requests of this launch
+            if (!block && !runners[i]->done()) return false;
             runners[i]->finish(&e, &g);
         } catch (const std::exception &ex) {
             err = ex.what();

@@ -45,6 +45,7 @@ _SIGS = {
     "lq_sv_device_ptr": (C.c_int, [_P, C.POINTER(_P)]),
     "lq_sv_apply_op": (C.c_int, [_P, C.c_char_p, _U32P, C.c_uint32, _U32P, _U8P, C.c_uint32, C.c_int, _DP,
                                  C.c_uint32]),
+    "lq_sv_apply_operations": (C.c_int, [_P, C.c_uint32, C.c_char_p, _U32P, _U32P, _U8P, _U32P, _DP]),
     "lq_sv_apply_matrix": (C.c_int, [_P, _P, _U32P, C.c_uint32, _U32P, _U8P, C.c_uint32, C.c_int]),
     "lq_sv_apply_sparse_matrix": (C.c_int, [_P, _I64P, C.c_uint64, _I64P, _P, C.c_uint64, _U32P, C.c_uint32, _U32P,
                                             _U8P, C.c_uint32, C.c_int]),

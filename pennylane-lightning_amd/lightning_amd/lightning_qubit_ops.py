@@ -204,6 +204,31 @@ class _StateVectorBase:
         if rc:
             check(rc)
 
+    def applyOperations(self, ops, wires, inverses, params=None):
+        """Named gates in order with one C call
+        (core/simulators/base/StateVectorBase.hpp:127-172)."""
+        n = len(ops)
+        if len(wires) != n or len(inverses) != n or (params is not None and len(params) != n):
+            raise LightningException("Invalid arguments: number of operations, wires, inverses, and parameters "
+                                     "must all be equal")
+        if n == 0:
+            return
+        blob = b"\0".join(_name_bytes(o) for o in ops) + b"\0"
+        woff = np.zeros(n + 1, dtype=np.uint32)
+        np.cumsum([len(w) for w in wires], out=woff[1:])
+        wflat = np.fromiter((x for w in wires for x in w), dtype=np.uint32, count=int(woff[-1]))
+        inv = np.fromiter((1 if x else 0 for x in inverses), dtype=np.uint8, count=n)
+        pp = pf = None
+        if params is not None:
+            poff = np.zeros(n + 1, dtype=np.uint32)
+            np.cumsum([len(p) for p in params], out=poff[1:])
+            pfl = np.fromiter((x for p in params for x in p), dtype=np.float64, count=int(poff[-1]))
+            pp = poff.ctypes.data_as(C.POINTER(C.c_uint32))
+            pf = pfl.ctypes.data_as(C.POINTER(C.c_double))
+        check(lib.lq_sv_apply_operations(self._h, n, blob, woff.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         wflat.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         inv.ctypes.data_as(C.POINTER(C.c_uint8)), pp, pf))
+
     def applyMatrix(self, matrix, wires, inverse=False):
         m = np.ascontiguousarray(np.asarray(matrix, dtype=self._cdtype).reshape(-1))
         w, wp, nw = u32(wires)

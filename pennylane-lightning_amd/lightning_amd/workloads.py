@@ -133,18 +133,31 @@ def qft_brickwork_mcx(n: int, layers: int = 8, mcx_per_layer: int = 1, seed: int
 
 
 def apply_circuit(sv, ops: list[CircuitOp]) -> None:
-    """Apply an op list to a StateVectorC* through its gate methods."""
+    """Apply an op list to a StateVectorC*: runs of plain named gates with
+    one ``applyOperations`` call each, matrices and controlled gates through
+    their own methods."""
+    run = []
+
+    def flush_run():
+        if run:
+            sv.applyOperations([o.name for o in run], [o.wires for o in run], [o.inverse for o in run],
+                               [o.params for o in run])
+            run.clear()
+
     for op in ops:
+        if op.matrix is None and not op.ctrls:
+            run.append(op)
+            continue
+        flush_run()
         if op.matrix is not None:
             if op.ctrls:
                 sv.applyControlledMatrix(op.matrix, op.ctrls, op.cvals or [True] * len(op.ctrls), op.wires,
                                          op.inverse)
             else:
                 sv.applyMatrix(op.matrix, op.wires, op.inverse)
-        elif op.ctrls:
-            getattr(sv, op.name)(op.ctrls, op.cvals or [True] * len(op.ctrls), op.wires, op.inverse, op.params)
         else:
-            getattr(sv, op.name)(op.wires, op.inverse, op.params)
+            getattr(sv, op.name)(op.ctrls, op.cvals or [True] * len(op.ctrls), op.wires, op.inverse, op.params)
+    flush_run()
 
 
 def to_ops(ops: list[CircuitOp], bits: int = 128):

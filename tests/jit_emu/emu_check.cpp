@@ -8,6 +8,7 @@
 //   CASE 0: hardware-efficient layers (Rot + CZ ladders) with named-generator dots
 //   CASE 1: the same plus controlled, parity, phase and SWAP gates
 //   CASE 2: case 1 on two states with dots between them (adjoint-like)
+//   CASE 3: hardware-efficient layers, then four named expectation values
 // Exit status 0 when every amplitude and dot matches (complex128 1e-10,
 // complex64 1e-3 relative).
 #include <dlfcn.h>
@@ -115,7 +116,22 @@ int main(int argc, char **argv) {
         prog.push_back(d);
     };
     if (which == 2) ns = 2;
-    for (int l = 0; l < 3; ++l) {
+    if (which == 3) {
+        // hardware-efficient layers, then <Z_0>, <X_{n-1}>, <Y_8>, <H_3> as the
+        // engine lowers named expectation values (one dot each)
+        for (int l = 0; l < 3; ++l) {
+            for (int w = 0; w < n; ++w) gate(GateId::Rot, {(uint32_t)w}, {U(rng), U(rng), U(rng)});
+            for (int s = 0; s < 2; ++s) for (int w = s; w < n - 1; w += 2) gate(GateId::CZ, {(uint32_t)w, (uint32_t)w + 1}, {});
+        }
+        const GateId obs[4] = {GateId::PauliZ, GateId::PauliX, GateId::PauliY, GateId::Hadamard};
+        const int ow[4] = {0, n - 1, 8, 3};
+        for (int i = 0; i < 4; ++i) {
+            LOp d = lower_matrix(gate_matrix(obs[i], nullptr, false, 1), {n - 1 - ow[i]}, 0, 0, true);
+            d.flags = (d.flags & ~TF_ZERO_UNCTRL) | TF_DOT; d.sa = 0; d.sb = 0; d.slot = nslots++;
+            prog.push_back(d);
+        }
+    }
+    for (int l = 0; l < (which == 3 ? 0 : 3); ++l) {
         for (int w = 0; w < n; ++w) gate(GateId::Rot, {(uint32_t)w}, {U(rng), U(rng), U(rng)});
         for (int s = 0; s < 2; ++s) for (int w = s; w < n - 1; w += 2) gate(GateId::CZ, {(uint32_t)w, (uint32_t)w + 1}, {});
         if (which >= 1) {
@@ -138,16 +154,26 @@ int main(int argc, char **argv) {
     // the engine's specialised-pass planning: lookahead tile bits, budget 52, 4 low bits
     cfg.lookahead = !(getenv("EMU_LOOKAHEAD") && atoi(getenv("EMU_LOOKAHEAD")) == 0);
     cfg.budget = getenv("EMU_NO_SPLIT") ? 24 : 52;
+    // EMU_BASIS=k (one state): the state is the basis state |k> not yet
+    // written (a deferred reset); as in the engine, the program's leading
+    // one-qubit gates are folded into its product factors and the first pass
+    // forms the product state, ignoring memory
+    const long long basis = (getenv("EMU_BASIS") && ns == 1) ? atoll(getenv("EMU_BASIS")) : -1;
+    std::vector<cd> fac;
+    std::vector<LOp> run_prog = prog;
+    if (basis >= 0) {
+        fac.assign(2 * static_cast<size_t>(n), cd(0.0, 0.0));
+        for (int q = 0; q < n; ++q) fac[2 * q + ((basis >> q) & 1)] = 1.0;
+        const size_t folded = absorb_into_product(run_prog, fac);
+        printf("folded %zu leading one-qubit ops into the product state\n", folded);
+    }
     // as the engine does for specialised passes: 2x2 gates as phase / real rotation / phase
-    std::vector<LOp> split = prog;
+    std::vector<LOp> split = run_prog;
     if (!getenv("EMU_NO_SPLIT")) split_1q_rotations(split);
     auto passes = plan_passes(split, cfg);
     // initial states: random normalised
     std::vector<St> S(ns, St(1ull << n));
     for (auto &s : S) for (auto &x : s) x = cd(U(rng) - 3.14, U(rng) - 3.14);
-    // EMU_BASIS=k (one state): the state is the basis state |k> not yet
-    // written (a deferred reset); the first pass must form it and ignore memory
-    const long long basis = (getenv("EMU_BASIS") && ns == 1) ? atoll(getenv("EMU_BASIS")) : -1;
     if (basis >= 0) {
         for (auto &x : S[0]) x = 0.0;
         S[0][static_cast<size_t>(basis)] = 1.0;
@@ -176,13 +202,23 @@ int main(int argc, char **argv) {
         void *h = dlopen((base + ".so").c_str(), RTLD_NOW | RTLD_LOCAL);
         if (!h) { printf("dlopen: %s\n", dlerror()); return 4; }
         Launch L = (Launch)dlsym(h, "emu_launch");
-        std::vector<double2> coef(passes[p].coef.size() + 1);
-        for (size_t i = 0; i + 1 < coef.size(); ++i) coef[i] = {passes[p].coef[i].real(), passes[p].coef[i].imag()};
-        // the entry after the coefficients: deferred reset marker, as the engine writes it
-        coef.back() = (p == 0 && basis >= 0) ? double2{1.0, static_cast<double>(basis)} : double2{-1.0, 0.0};
+        // coefficients, then the deferred-preparation marker and the product
+        // factors, as the engine writes them
+        const size_t nc = passes[p].coef.size();
+        std::vector<double2> coef(nc + 1 + 2 * static_cast<size_t>(n), double2{0.0, 0.0});
+        for (size_t i = 0; i < nc; ++i) coef[i] = {passes[p].coef[i].real(), passes[p].coef[i].imag()};
+        const bool form = p == 0 && basis >= 0;
+        if (form && !((passes[p].load_mask & 1u) && (passes[p].store_mask & 1u))) {
+            printf("first pass cannot form the product state\n");
+            return 5;
+        }
+        coef[nc] = form ? double2{1.0, 0.0} : double2{-1.0, 0.0};
+        if (form)
+            for (size_t i = 0; i < fac.size(); ++i) coef[nc + 1 + i] = {fac[i].real(), fac[i].imag()};
         const uint64_t ntiles = 1ull << (n - (int)passes[p].tbits.size());
         std::vector<double2> part(std::max<size_t>(1, passes[p].slot_ids.size() * ntiles), {0, 0});
-        const unsigned grid = jit_config().persistent ? (unsigned)std::min<uint64_t>(ntiles, grid_cap) : (unsigned)ntiles;
+        const unsigned grid = (unsigned)ntiles;
+        (void)grid_cap;
         auto t0 = std::chrono::steady_clock::now();
         L(buf[0].data(), ns > 1 ? buf[1].data() : nullptr, coef.data(), part.data(), ntiles, grid);
         double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -2,7 +2,8 @@
 the engine's planner and kernel generator in liblq_amd.so produce each pass
 of three programs (hardware-efficient layers with dots; plus controlled,
 parity, phase and SWAP gates; the same on two states with dots between them,
-as in the adjoint sweep), the kernels are compiled for the host, run with one
+as in the adjoint sweep; and layers followed by named expectation values),
+the kernels are compiled for the host, run with one
 host thread per GPU thread, and compared with a naive application of the same
 ops (complex128 within 1e-10, complex64 within 1e-3 relative)."""
 import os
@@ -33,22 +34,24 @@ def emu_binary(tmp_path_factory):
     return exe, str(d)
 
 
-@pytest.mark.parametrize("case,dtype,persistent,lds32,look", [
-    (0, 1, 0, 0, 1), (1, 1, 0, 0, 1), (2, 1, 0, 0, 1), (1, 0, 0, 0, 1), (2, 0, 0, 0, 1), (2, 1, 1, 0, 1),
-    (0, 1, 0, 1, 1), (2, 1, 0, 1, 1), (2, 0, 0, 1, 1), (0, 1, 0, 0, 0), (1, 0, 0, 0, 0), (2, 1, 0, 0, 0),
-    (0, 1, 0, 1, 2), (1, 0, 0, 1, 2), (2, 1, 0, 1, 2), (0, 1, 0, 1, 3), (1, 0, 0, 1, 3)])
-def test_generated_kernels_match_naive_application(emu_binary, case, dtype, persistent, lds32, look):
+@pytest.mark.parametrize("case,dtype,lds32,look,n", [
+    (0, 1, 0, 1, 14), (1, 1, 0, 1, 14), (2, 1, 0, 1, 14), (1, 0, 0, 1, 14), (2, 0, 0, 1, 14),
+    (0, 1, 1, 1, 14), (2, 1, 1, 1, 14), (2, 0, 1, 1, 14), (0, 1, 0, 0, 14), (1, 0, 0, 0, 14), (2, 1, 0, 0, 14),
+    (0, 1, 1, 2, 16), (1, 0, 1, 2, 16), (2, 1, 1, 2, 16), (0, 1, 1, 3, 14), (1, 0, 1, 3, 14),
+    (3, 0, 1, 1, 17), (3, 0, 1, 3, 17), (3, 1, 1, 3, 17)])
+def test_generated_kernels_match_naive_application(emu_binary, case, dtype, lds32, look, n):
     exe, work = emu_binary
     # lds32: transposes through 32 KiB of LDS (emulated at exactly the size the kernel declares);
     # look 0: tile bits from the first ready op instead of the planner's lookahead;
     # look 2: lookahead, and loads/stores in the first/last stage's mapping wherever they coalesce
     # (16 qubits: passes whose first and last stages leave the low bits to the lanes)
-    # look 3: lookahead, and the state a basis state not yet written (deferred reset, formed by the first pass)
-    env = dict(os.environ, LQ_JIT_PERSIST=str(persistent), LQ_JIT_LDS32=str(lds32), EMU_LOOKAHEAD=str(min(look, 1)),
+    # look 3: lookahead, and the state a basis state not yet written (deferred reset): the leading
+    # one-qubit gates are folded into product factors and the first pass forms the product state
+    # case 3 at 17 qubits and complex64: the program of the complex64 expval GPU test
+    env = dict(os.environ, LQ_JIT_LDS32=str(lds32), EMU_LOOKAHEAD=str(min(look, 1)),
                LQ_JIT_DIRECT="2" if look == 2 else "1")
     if look == 3:
         env["EMU_BASIS"] = "5"
-    # 14 qubits: 4 tiles per pass, walked by 3 workgroups in the persistent form
-    r = subprocess.run([exe, str(case), str(dtype), "16" if look == 2 else "14", "3", EMU, work], capture_output=True, text=True,
-                       timeout=600, env=env)
+    r = subprocess.run([exe, str(case), str(dtype), str(n), "3", EMU, work], capture_output=True, text=True,
+                       timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
