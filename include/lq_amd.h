@@ -101,6 +101,10 @@ int lq_sv_apply_generator(lq_sv_t sv, const char *name, const uint32_t *wires, u
                           const uint8_t *ctrl_vals, uint32_t nc, int adj, double *scale);
 /* normalize / collapse (lq/StateVectorLQubit.hpp:881-921) */
 int lq_sv_normalize(lq_sv_t sv);
+/* <psi|psi> of the state after its queued gates, FP64 compensated sum (the
+ * norm computeNorm of normalize, lq/StateVectorLQubit.hpp:908-921); for a
+ * complex64 state its distance from 1 is the rounding drift of the run */
+int lq_sv_norm2(lq_sv_t sv, double *out);
 int lq_sv_collapse(lq_sv_t sv, uint32_t wire, int branch);
 
 /* ---- Measurements (lq/measurements/MeasurementsLQubit.hpp) ---------------- */

@@ -201,6 +201,13 @@ int lq_sv_normalize(lq_sv_t sv) {
     return guard([&] { S(sv).normalize(); });
 }
 
+int lq_sv_norm2(lq_sv_t sv, double *out) {
+    return guard([&] {
+        LQ_ABORT_IF(!out, "null argument");
+        *out = S(sv).norm2();
+    });
+}
+
 int lq_sv_collapse(lq_sv_t sv, uint32_t wire, int branch) {
     return guard([&] { S(sv).collapse(wire, branch != 0); });
 }
@@ -308,7 +315,7 @@ int lq_obs_named(const char *name, const uint32_t *wires, uint32_t nw, lq_obs_t 
 
 int lq_obs_hermitian(const void *matrix_c128, const uint32_t *wires, uint32_t nw, lq_obs_t *out) {
     return guard([&] {
-        LQ_ABORT_IF(nw < 1 || nw > 10, "Hermitian observables are supported on 1 to 10 wires");
+        LQ_ABORT_IF(nw < 1 || nw > 30, "Hermitian observables are supported on 1 to 30 wires");
         auto o = std::make_shared<Obs>();
         o->kind = Obs::Hermitian;
         o->wires.assign(wires, wires + nw);

@@ -732,7 +732,6 @@ void StateVector::apply_named(const std::string &name, const uint32_t *wires, in
 void StateVector::apply_matrix(const void *mat, const uint32_t *wires, int nw, const uint32_t *ctrls,
                                const uint8_t *cvals, int nc, bool inverse) {
     LQ_ABORT_IF(nw < 1, "Number of wires must be larger than 0");
-    LQ_ABORT_IF(nw > 10, "dense matrices are supported on at most 10 wires");
     check_wires(n_, wires, nw, ctrls, nc);
     const size_t d = static_cast<size_t>(1) << nw;
     std::vector<cd> m(d * d);
@@ -756,50 +755,58 @@ void StateVector::apply_matrix(const void *mat, const uint32_t *wires, int nw, c
         cm |= 1ull << phys_bit(n_, ctrls[i]);
         if (!cvals || cvals[i]) cv |= 1ull << phys_bit(n_, ctrls[i]);
     }
+    if (nw > kMaxFusedTargets) {
+        // wider than a tile op: the pending gates, then the operator out of
+        // place into a work state, copied back
+        flush();
+        void *tmp = workspace(5);
+        apply_wide(std::move(m), tb, cm, cv, data(), tmp);
+        launch_stream_copy(tmp, d_, bytes(), stream_);
+        return;
+    }
     enqueue(lower_matrix(std::move(m), tb, cm, cv, false));
+}
+
+void StateVector::apply_wide(std::vector<cd> m, const std::vector<int> &tb, uint64_t cm, uint64_t cv, const void *in,
+                             void *out) {
+    const int k = static_cast<int>(tb.size());
+    LQ_ABORT_IF(m.size() != (static_cast<size_t>(1) << (2 * k)), "The size of matrix does not match with the given number of wires");
+    std::vector<double2> h(m.size());
+    for (size_t i = 0; i < m.size(); ++i) h[i] = make_double2(m[i].real(), m[i].imag());
+    // the previous upload may still be read by a kernel in flight
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    wide_.ensure(h.size() * sizeof(double2));
+    hip_check(hipMemcpyAsync(wide_.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, stream_),
+              "hipMemcpyAsync(matrix)");
+    dispatch(dtype_, [&](auto t) {
+        launch_dense_wide<decltype(t)>(in, out, static_cast<const double2 *>(wide_.p), tb.data(), k, cm, cv, n_,
+                                       stream_);
+    });
+    hip_check(hipGetLastError(), "dense operator launch");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); // h is freed on return
 }
 
 void StateVector::apply_pauli_rot(const uint32_t *wires, int nw, bool inverse, double theta, const std::string &word) {
     LQ_ABORT_IF(static_cast<int>(word.size()) != nw, "wires and word have incompatible dimensions.");
     check_wires(n_, wires, nw, nullptr, 0);
-    // exp(-i theta/2 P): an all-Z word is MultiRZ; any letter other than Z
-    // and Y flips its bit like X (GateImplementationsLM.hpp:575-629)
-    if (std::all_of(word.begin(), word.end(), [](char ch) { return ch == 'Z'; })) {
-        enqueue(lower_gate(n_, GateId::MultiRZ, wires, nw, nullptr, nullptr, 0, inverse, &theta));
-        return;
-    }
-    PauliWord w{1.0, 0, 0, 0};
+    // exp(-i theta/2 P) = V^dagger MultiRZ(theta) V with V the per-wire basis
+    // change taking P to a product of Z: H for X, H S^dagger for Y, nothing
+    // for Z; every letter other than Z and Y acts as X, as in the reference
+    // (GateImplementationsLM.hpp:575-629). An all-Z word is MultiRZ alone.
+    auto one = [&](GateId g, uint32_t w, bool inv) {
+        enqueue(lower_gate(n_, g, &w, 1, nullptr, nullptr, 0, inv, nullptr));
+    };
     for (int j = 0; j < nw; ++j) {
-        const uint64_t b = 1ull << phys_bit(n_, wires[j]);
-        if (word[j] == 'Z') w.yz |= b;
-        else if (word[j] == 'Y') {
-            w.x |= b;
-            w.yz |= b;
-            ++w.ny;
-        } else w.x |= b;
+        if (word[j] == 'Z') continue;
+        if (word[j] == 'Y') one(GateId::S, wires[j], true);
+        one(GateId::Hadamard, wires[j], false);
     }
-    // cos(t/2) I - i sin(t/2) P as a dense op on the word's bits
-    std::vector<int> tb;
-    for (int j = 0; j < nw; ++j) tb.push_back(phys_bit(n_, wires[j]));
-    LQ_ABORT_IF(nw > 10, "PauliRot is supported on at most 10 wires");
-    const size_t d = static_cast<size_t>(1) << nw;
-    std::vector<cd> m(d * d, 0.0);
-    const double t = inverse ? -theta : theta;
-    const double c = std::cos(t / 2), s = std::sin(t / 2);
-    cd ph = 1.0;
-    for (int i = 0; i < (w.ny & 3); ++i) ph *= cd(0, -1);
-    for (size_t col = 0; col < d; ++col) {
-        m[col * d + col] += c;
-        // P|col>: flip x bits, sign from y/z bits of the row index
-        size_t row = col;
-        for (int j = 0; j < nw; ++j)
-            if (word[j] != 'Z') row ^= static_cast<size_t>(1) << (nw - 1 - j);
-        int sg = 0;
-        for (int j = 0; j < nw; ++j)
-            if ((word[j] == 'Y' || word[j] == 'Z') && ((row >> (nw - 1 - j)) & 1)) sg ^= 1;
-        m[row * d + col] += cd(0, -s) * ph * (sg ? -1.0 : 1.0);
+    enqueue(lower_gate(n_, GateId::MultiRZ, wires, nw, nullptr, nullptr, 0, inverse, &theta));
+    for (int j = 0; j < nw; ++j) {
+        if (word[j] == 'Z') continue;
+        one(GateId::Hadamard, wires[j], false);
+        if (word[j] == 'Y') one(GateId::S, wires[j], false);
     }
-    enqueue(lower_matrix(std::move(m), tb, 0, 0, false));
 }
 
 double StateVector::apply_generator(const std::string &name, const uint32_t *wires, int nw, const uint32_t *ctrls,
@@ -974,7 +981,7 @@ std::vector<double> StateVector::expval_named_batch(const std::vector<std::strin
 }
 
 double StateVector::expval_matrix(const void *mat, const uint32_t *wires, int nw) {
-    LQ_ABORT_IF(nw < 1 || nw > 10, "matrix observables are supported on 1 to 10 wires");
+    LQ_ABORT_IF(nw < 1, "Number of wires must be larger than 0");
     check_wires(n_, wires, nw, nullptr, 0);
     const size_t d = static_cast<size_t>(1) << nw;
     std::vector<cd> m(d * d);
@@ -987,6 +994,12 @@ double StateVector::expval_matrix(const void *mat, const uint32_t *wires, int nw
     }
     std::vector<int> tb(nw);
     for (int j = 0; j < nw; ++j) tb[j] = phys_bit(n_, wires[j]);
+    if (nw > kMaxFusedTargets) {
+        flush();
+        void *tmp = workspace(5);
+        apply_wide(std::move(m), tb, 0, 0, data(), tmp);
+        return inner(d_, tmp).real();
+    }
     std::vector<LOp> prog;
     prog.swap(queue_);
     prog.push_back(make_dot(lower_matrix(std::move(m), tb, 0, 0, true), 0, 0, 0));
@@ -1079,9 +1092,29 @@ void StateVector::apply_obs_to(const Obs &o, const void *in, void *out) {
         }
         return;
     }
-    hip_check(hipMemcpyAsync(out, in, bytes(), hipMemcpyDeviceToDevice, stream_), "memcpy");
+    launch_stream_copy(in, out, bytes(), stream_);
+    // factors act in place as one program; a Hermitian factor wider than a
+    // tile op runs out of place through a work state between them
+    std::vector<const Obs *> factors;
+    if (o.kind == Obs::Tensor)
+        for (const auto &f : o.terms) factors.push_back(f.get());
+    else
+        factors.push_back(&o);
     std::vector<LOp> prog;
-    apply_obs_inplace(o, out, prog);
+    for (const Obs *f : factors) {
+        if (f->kind == Obs::Hermitian && static_cast<int>(f->wires.size()) > kMaxFusedTargets) {
+            run(prog, {out}, 0);
+            prog.clear();
+            std::vector<int> tb;
+            for (auto w : f->wires) tb.push_back(phys_bit(n_, w));
+            void *tmp = workspace(8);
+            LQ_ABORT_IF(tmp == in || tmp == out, "workspace clash");
+            apply_wide(f->matrix, tb, 0, 0, out, tmp);
+            launch_stream_copy(tmp, out, bytes(), stream_);
+        } else {
+            apply_obs_inplace(*f, out, prog);
+        }
+    }
     run(prog, {out}, 0);
 }
 
@@ -1148,10 +1181,21 @@ double StateVector::expval_obs(const Obs &o) {
 // Variances, sparse Hamiltonians, samples
 // ---------------------------------------------------------------------------
 void StateVector::dense_moments(std::vector<cd> m, const uint32_t *wires, int nw, double *mean, double *mean_square) {
-    LQ_ABORT_IF(nw < 0 || nw > 10, "matrix observables are supported on 1 to 10 wires");
+    LQ_ABORT_IF(nw < 0, "Number of wires must not be negative");
     check_wires(n_, wires, nw, nullptr, 0);
     const size_t d = static_cast<size_t>(1) << nw;
     LQ_ABORT_IF(m.size() != d * d, "The size of matrix does not match with the given number of wires");
+    if (nw > kMaxFusedTargets) {
+        // O psi into a work state: Re<psi|O psi> and <O psi|O psi>
+        std::vector<int> tb(static_cast<size_t>(nw));
+        for (int j = 0; j < nw; ++j) tb[j] = phys_bit(n_, wires[j]);
+        flush();
+        void *tmp = workspace(5);
+        apply_wide(std::move(m), tb, 0, 0, data(), tmp);
+        *mean = inner(d_, tmp).real();
+        *mean_square = inner(tmp, tmp).real();
+        return;
+    }
     // <O psi|O psi> = <psi|O^dagger O|psi>
     std::vector<cd> mm(d * d, 0.0);
     for (size_t r = 0; r < d; ++r)
@@ -1199,7 +1243,7 @@ double StateVector::var_named(const std::string &name, const uint32_t *wires, in
 }
 
 double StateVector::var_matrix(const void *mat, const uint32_t *wires, int nw) {
-    LQ_ABORT_IF(nw < 1 || nw > 10, "matrix observables are supported on 1 to 10 wires");
+    LQ_ABORT_IF(nw < 1, "Number of wires must be larger than 0");
     const size_t d = static_cast<size_t>(1) << nw;
     std::vector<cd> m(d * d);
     if (dtype_) {

@@ -87,6 +87,10 @@ struct OpsList {
     size_t num_par_ops = 0;
 };
 
+// Dense operators on more targets than this run out of place through a
+// global-memory kernel instead of as tile ops.
+constexpr int kMaxFusedTargets = 10;
+
 struct EngineTuning {
     int kt_max = 0;      // 0: from the LDS budget
     int low_bits = -1;   // -1: 4 (complex128) / 5 (complex64); specialised passes 4
@@ -141,6 +145,10 @@ class StateVector {
     double apply_generator(const std::string &name, const uint32_t *wires, int nw, const uint32_t *ctrls,
                            const uint8_t *cvals, int nc);
     void apply_oprec(const OpRec &op, bool adj);
+    // out = M in for a dense operator on physical bits tb (any width; out of
+    // place, in != out); controls (cm, cv) keep other amplitudes unchanged
+    void apply_wide(std::vector<cd> m, const std::vector<int> &tb, uint64_t cm, uint64_t cv, const void *in,
+                    void *out);
     void normalize();
     void collapse(uint32_t wire, bool branch);
 
@@ -166,6 +174,7 @@ class StateVector {
                           uint64_t *out);
 
     // queue
+    bool has_queued() const { return !queue_.empty(); }
     void enqueue(LOp op);
     void flush();
     void synchronize();
@@ -246,6 +255,7 @@ class StateVector {
     void sparse_moments(const CsrView &h, double *mean, double *mean_square);
     // CSR operator on the device: offsets, then column indices, then values
     DevBuf sparse_;
+    DevBuf wide_; // matrix of the last operator wider than a tile op
     size_t csr_indices_off_ = 0, csr_values_off_ = 0;
     std::vector<char> sparse_host_;
     CsrView sparse_view(const Obs &o);

@@ -349,6 +349,40 @@ __global__ __launch_bounds__(kBlock) void stream_copy_kernel(const double2 *__re
     }
 }
 
+// Dense 2^k x 2^k operator on k target bits (any k), out of place: one
+// thread per output amplitude sums its matrix row against the 2^k amplitudes
+// of its group (targets bits[0] = local MSB). Amplitudes whose control bits
+// do not match are copied unchanged. M is row-major complex double.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dense_wide_kernel(const typename VecT<T>::type *__restrict__ in,
+                                                           typename VecT<T>::type *__restrict__ out,
+                                                           const double2 *__restrict__ M, BitList bits, int k,
+                                                           uint64_t tmask, uint64_t cmask, uint64_t cval, uint64_t N) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    const uint64_t d = 1ull << k;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < N; i += stride) {
+        if ((i & cmask) != cval) {
+            out[i] = in[i];
+            continue;
+        }
+        uint64_t r = 0;
+        for (int j = 0; j < k; ++j) r |= ((i >> bits.b[j]) & 1ull) << (k - 1 - j);
+        const uint64_t base = i & ~tmask;
+        const double2 *row = M + r * d;
+        double ar = 0.0, ai = 0.0;
+        for (uint64_t c = 0; c < d; ++c) {
+            uint64_t idx = base; // the group's amplitude with target bits c
+            for (int j = 0; j < k; ++j) idx |= ((c >> (k - 1 - j)) & 1ull) << bits.b[j];
+            const typename VecT<T>::type v = in[idx];
+            const double2 m = row[c];
+            const double vx = v.x, vy = v.y;
+            ar += m.x * vx - m.y * vy;
+            ai += m.x * vy + m.y * vx;
+        }
+        out[i] = mk<T>(static_cast<T>(ar), static_cast<T>(ai));
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void scale_kernel(typename VecT<T>::type *a, uint64_t N, double re, double im) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
@@ -543,6 +577,21 @@ void launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t 
     hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(kBlock), 0, s, static_cast<const double2 *>(src),
                        static_cast<double2 *>(dst), N);
 }
+
+template <typename T>
+void launch_dense_wide(const void *in, void *out, const double2 *M, const int *bits, int k, uint64_t cmask,
+                       uint64_t cval, int n, hipStream_t s) {
+    using V = typename VecT<T>::type;
+    const uint64_t N = 1ull << n;
+    uint64_t tmask = 0;
+    for (int j = 0; j < k; ++j) tmask |= 1ull << bits[j];
+    hipLaunchKernelGGL(dense_wide_kernel<T>, dim3(stream_blocks(N)), dim3(kBlock), 0, s, static_cast<const V *>(in),
+                       static_cast<V *>(out), M, to_bitlist(bits, k), k, tmask, cmask, cval, N);
+}
+template void launch_dense_wide<double>(const void *, void *, const double2 *, const int *, int, uint64_t, uint64_t,
+                                        int, hipStream_t);
+template void launch_dense_wide<float>(const void *, void *, const double2 *, const int *, int, uint64_t, uint64_t,
+                                       int, hipStream_t);
 
 template <typename T> void launch_scale(void *a, int n, double re, double im, hipStream_t s) {
     using V = typename VecT<T>::type;

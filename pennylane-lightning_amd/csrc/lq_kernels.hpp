@@ -98,6 +98,12 @@ struct ProductFactors {
 template <typename T> void launch_fill_product(void *a, int n, const ProductFactors &f, hipStream_t s);
 // dst = src, bytes a multiple of 16 (nontemporal 16-byte loads and stores)
 void launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
+// out = M in for a dense 2^k x 2^k operator on physical bits[0..k) (bits[0]
+// the local MSB; M row-major complex double on the device), amplitudes whose
+// controls do not match copied unchanged; in != out
+template <typename T>
+void launch_dense_wide(const void *in, void *out, const double2 *M, const int *bits, int k, uint64_t cmask,
+                       uint64_t cval, int n, hipStream_t s);
 template <typename T> void launch_scale(void *a, int n, double re, double im, hipStream_t s);
 // zero amplitudes whose bit `bit` != branch, then scale the rest by `factor`
 template <typename T>

@@ -53,6 +53,7 @@ _SIGS = {
     "lq_sv_apply_generator": (C.c_int, [_P, C.c_char_p, _U32P, C.c_uint32, _U32P, _U8P, C.c_uint32, C.c_int,
                                         _DP]),
     "lq_sv_normalize": (C.c_int, [_P]),
+    "lq_sv_norm2": (C.c_int, [_P, _DP]),
     "lq_sv_collapse": (C.c_int, [_P, C.c_uint32, C.c_int]),
     "lq_probs": (C.c_int, [_P, _U32P, C.c_uint32, C.c_int, _DP]),
     "lq_expval_named": (C.c_int, [_P, C.c_char_p, _U32P, C.c_uint32, _DP]),

@@ -283,6 +283,12 @@ class _StateVectorBase:
     def normalize(self):
         check(lib.lq_sv_normalize(self._h))
 
+    def norm2(self) -> float:
+        """<psi|psi> in FP64 on the device (queued gates run first)."""
+        out = C.c_double()
+        check(lib.lq_sv_norm2(self._h, C.byref(out)))
+        return out.value
+
     def collapse(self, wire, branch):
         check(lib.lq_sv_collapse(self._h, int(wire), int(bool(branch))))
 

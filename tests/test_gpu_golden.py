@@ -47,6 +47,26 @@ def test_rxyz_cnot_n20():
     np.testing.assert_allclose(st[f["sample_idx"]], f["sample_amps"], atol=1e-12)
 
 
+@pytest.mark.parametrize("bits", [128, 64])
+def test_rxyz_cnot_n20_full_vectors(bits):
+    """The whole 2^20 state and probability vector against the oracle (pinned
+    to the compiled reference; its block sums and sampled amplitudes match the
+    fixture above): complex128 within 1e-12, complex64 within its stated
+    bound."""
+    from helpers import oracle_run
+
+    f = load("rxyz_cnot_n20_l10.npz")
+    n = 20
+    ops = G.to_circuit(G.rxyz_cnot_ops(f["angles"]))
+    exp = oracle_run(n, ops)
+    np.testing.assert_allclose((np.abs(exp) ** 2).reshape(1024, -1).sum(1), f["probs_block_sums"], atol=1e-14)
+    sv = run(n, G.rxyz_cnot_ops(f["angles"]), bits)
+    m = (L.MeasurementsC128 if bits == 128 else L.MeasurementsC64)(sv)
+    tol_a, tol_p = (1e-12, 1e-12) if bits == 128 else (5e-6, 1e-9)
+    np.testing.assert_allclose(m.probs(), np.abs(exp) ** 2, atol=tol_p)
+    np.testing.assert_allclose(sv.state(), exp, atol=tol_a)
+
+
 def test_hea_n12_both_precisions():
     f = load("hea_rot_cz_n12_l4.npz")
     n = 12
