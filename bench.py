@@ -338,6 +338,69 @@ def run_qft33(args):
     print(json.dumps(out))
 
 
+def run_mfma5(args):
+    """Matrix cores for fused dense blocks, complex64: one Haar-random 32x32
+    unitary on wires 25..29 of a 30-qubit complex64 state (the five lowest
+    index bits), applied (a) as a 64x64 real GEMM per 32-amplitude group on
+    the MFMA units, (b) as a dense tile op in the engine's default path, and
+    (c) the gates such a block fuses in the hardware-efficient ansatz (one
+    layer on those wires: 5 Rot + 4 CZ) through the engine's specialised
+    passes. Times per application (stream-synchronised), HBM rate over
+    8 GiB read + 8 GiB written, and the GEMM's fp32 rate (256 flop per
+    amplitude)."""
+    from lightning_amd import lightning_qubit_ops as L
+
+    n = args.qubits
+    rng = np.random.default_rng(3)
+    z = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+    q, r = np.linalg.qr(z)
+    u = (q * (np.diag(r) / np.abs(np.diag(r)))).astype(np.complex64)
+    wires = list(range(n - 5, n))
+    sv = L.StateVectorC64(n)
+    sv.resetStateVector()
+    sv.synchronize()
+    S = (1 << n) * 8
+    reps = max(args.steps, 3)
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        sv.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        sv.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    def dense():
+        sv.applyMatrix(u, wires, False)
+        sv.synchronize()
+
+    L.set_mfma_dense5(True)
+    ms_mfma = timed(dense)
+    L.set_mfma_dense5(False)
+    ms_tile = timed(dense)
+    ang = rng.uniform(0, 2 * np.pi, size=(5, 3))
+
+    def layer():
+        for j, w in enumerate(wires):
+            sv.Rot([w], False, [float(x) for x in ang[j]])
+        for a in (n - 5, n - 3, n - 4, n - 2):
+            sv.CZ([a, a + 1], False, [])
+        sv.synchronize()
+
+    layer()
+    L.jit_wait()
+    ms_gates = timed(layer)
+    out = {"metric": "complex64 fused 2^5 x 2^5 block: matrix cores vs engine paths", "n_qubits": n,
+           "dtype": "complex64", "n_gpus": 1, "data": "synthetic", "reps": reps,
+           "mfma_gemm": {"ms": round(ms_mfma, 3), "GBps": round(2 * S / ms_mfma / 1e6, 1),
+                         "fp32_TFLOPs": round(256 * (1 << n) / ms_mfma / 1e9, 1)},
+           "dense_tile_op": {"ms": round(ms_tile, 3), "GBps": round(2 * S / ms_tile / 1e6, 1)},
+           "factored_gates_5rot_4cz": {"ms": round(ms_gates, 3), "GBps": round(2 * S / ms_gates / 1e6, 1)}}
+    print(json.dumps(out))
+
+
 def run_precision(args):
     """complex64 against complex128 for each workload: state fidelity and
     output errors (expvals, probabilities, energies, gradients)."""
@@ -416,7 +479,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain streaming kernel (4 GiB read + 4 GiB written per launch)")
-    ap.add_argument("--workload", choices=["hea30", "adjoint24", "serving", "qft33", "precision"], default="hea30",
+    ap.add_argument("--workload", choices=["hea30", "adjoint24", "serving", "qft33", "precision", "mfma5"],
+                    default="hea30",
                     help="hea30: the gate-application line (default); adjoint24: the 24-qubit adjoint gradient; "
                          "serving: energy + gradient requests, batched vs plain; qft33: 33-qubit QFT + Haar "
                          "brickwork + 3-controlled X with complex64 fidelity; precision: complex64 error per "
@@ -453,6 +517,8 @@ def main():
         return run_qft33(args)
     if args.workload == "precision":
         return run_precision(args)
+    if args.workload == "mfma5":
+        return run_mfma5(args)
     from lightning_amd import lightning_qubit_ops as L
     from lightning_amd import workloads as W
 
@@ -488,9 +554,14 @@ def main():
     sv.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    recs = sv.profile_passes()
     st = sv.profile_read(reset=True)
     sv.profile(False)
     pc = sv.pass_counts(reset=True)
+    per_step = int(round(st["passes"] / args.steps)) if args.steps else 0
+    # the last step's passes, in launch order
+    pass_table = [[round(r["ms"], 3), round(r["bytes"] / 1e9, 2), r["ops"], r["stages"], r["transposes"],
+                   int(r["specialised"]), r["tile_bits"]] for r in (recs[-per_step:] if per_step else [])]
     real = "double" if elem == 16 else "float"
     if pc["generic"] == 0:
         kernel = "lqpass (specialised register tile pass, hiprtc)"
@@ -541,6 +612,8 @@ def main():
                      "specialised_passes": pc["specialised"], "generic_passes": pc["generic"],
                      "copy_kernel_GBps": round(copy_bps / 1e9, 1),
                      "frac_of_copy": round(achieved / copy_bps, 4) if copy_bps else None},
+        "pass_table": {"columns": ["kernel_ms", "GB", "ops", "stages", "lds_transposes", "specialised", "tile_bits"],
+                       "rows": pass_table},
         "gates_per_s": round(len(ops) / (ms_step * 1e-3), 1),
         "expvals": {"Z0Z1": ev[0], f"X{n-1}": ev[1]},
         "cpu_baseline": cpu,

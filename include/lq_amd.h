@@ -192,6 +192,10 @@ int lq_vector_jacobian_product(lq_sv_t sv, lq_ops_t ops, const void *dy, uint64_
 int lq_sv_profile(lq_sv_t sv, int enable);
 int lq_sv_profile_read(lq_sv_t sv, int reset, uint64_t *passes, double *ms, double *bytes, double *max_ms);
 int lq_sv_stream(lq_sv_t sv, void **out);
+/* Per-pass records of the profiled passes since the last reset: up to `cap`
+ * rows of 7 doubles (kernel ms, algorithmic bytes, ops, register stages, LDS
+ * transposes, 1 if specialised, tile-bit mask); *count = rows available. */
+int lq_sv_profile_passes(lq_sv_t sv, double *out, uint64_t cap, uint64_t *count);
 
 /* <a|b> of two states of the same size, either precision, accumulated in FP64
  * (no reference counterpart; used for the complex64 fidelity reports) */
@@ -259,6 +263,22 @@ int lq_jit_shutdown(void);
 int lq_jit_stats(int *ready, int *failed, int *pending);
 int lq_sv_pass_counts(lq_sv_t sv, int reset, uint64_t *specialised, uint64_t *generic);
 int lq_set_split_fallback(int mode, int *previous);
+/* Matrix-core path for complex64 dense operators on exactly the five
+ * lowest index bits (wires n-5 .. n-1 in any order) without controls:
+ * with on != 0, applyMatrix of such an operator runs as one 64x64 real
+ * GEMM per 32-amplitude group on the MFMA units (v_mfma_f32_32x32x2_f32)
+ * instead of as a fused tile op; default off (LQ_MFMA_DENSE5). Replaces the
+ * dense k-qubit kernel for that case (lq/gates/cpu_kernels/
+ * GateImplementationsLM.hpp:454-498, applyNQubitOp). */
+int lq_set_mfma_dense5(int on, int *previous);
+/* Engine options by name (process-wide; *previous may be NULL):
+ * "split_fallback" (0..2, as lq_set_split_fallback), "mfma_dense5" (0/1, as
+ * lq_set_mfma_dense5), "product_fold" (0/1: after a reset the leading
+ * one-qubit gates of the next program fold into the product state its first
+ * pass forms; default 0, LQ_PRODUCT_FOLD), "plan_search" (0/1: states of 26
+ * qubits and more plan specialised programs under several pass budgets and
+ * keep the plan with the fewest passes; default 0, LQ_PLAN_SEARCH). */
+int lq_set_engine_option(const char *name, int value, int *previous);
 
 #ifdef __cplusplus
 }

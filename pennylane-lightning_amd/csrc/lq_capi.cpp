@@ -447,6 +447,25 @@ int lq_sv_profile_read(lq_sv_t sv, int reset, uint64_t *passes, double *ms, doub
     });
 }
 
+int lq_sv_profile_passes(lq_sv_t sv, double *out, uint64_t cap, uint64_t *count) {
+    return guard([&] {
+        const auto recs = S(sv).profile_passes();
+        if (count) *count = recs.size();
+        LQ_ABORT_IF(cap && !out, "null argument");
+        for (size_t i = 0; i < recs.size() && i < cap; ++i) {
+            const auto &r = recs[i];
+            double *o = out + 7 * i;
+            o[0] = r.ms;
+            o[1] = r.bytes;
+            o[2] = r.ops;
+            o[3] = r.stages;
+            o[4] = r.transposes;
+            o[5] = r.specialised;
+            o[6] = static_cast<double>(r.tile_bits);
+        }
+    });
+}
+
 int lq_sv_stream(lq_sv_t sv, void **out) {
     return guard([&] { *out = static_cast<void *>(S(sv).stream()); });
 }
@@ -609,6 +628,37 @@ int lq_set_split_fallback(int mode, int *previous) {
     return guard([&] {
         LQ_ABORT_IF(mode < 0 || mode > 2, "split fallback mode must be 0, 1 or 2");
         const int old = tuning().split_fallback.exchange(mode);
+        if (previous) *previous = old;
+    });
+}
+
+int lq_set_engine_option(const char *name, int value, int *previous) {
+    return guard([&] {
+        LQ_ABORT_IF(name == nullptr, "null argument");
+        const std::string k(name);
+        EngineTuning &t = tuning();
+        std::atomic<int> *opt = nullptr;
+        if (k == "split_fallback") {
+            LQ_ABORT_IF(value < 0 || value > 2, "split fallback mode must be 0, 1 or 2");
+            opt = &t.split_fallback;
+        } else if (k == "mfma_dense5") {
+            opt = &t.mfma_dense5;
+        } else if (k == "product_fold") {
+            opt = &t.product_fold;
+        } else if (k == "plan_search") {
+            opt = &t.plan_search;
+        } else {
+            LQ_ABORT((std::string("unknown engine option ") + k).c_str());
+        }
+        if (k != "split_fallback") value = value ? 1 : 0;
+        const int old = opt->exchange(value);
+        if (previous) *previous = old;
+    });
+}
+
+int lq_set_mfma_dense5(int on, int *previous) {
+    return guard([&] {
+        const int old = tuning().mfma_dense5.exchange(on ? 1 : 0);
         if (previous) *previous = old;
     });
 }

@@ -66,6 +66,9 @@ EngineTuning &tuning() {
         if (const char *s = std::getenv("LQ_PLAN_CACHE")) e.plan_cache = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_PLAN_LOOKAHEAD")) e.lookahead = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_SPLIT_FALLBACK")) e.split_fallback.store(std::atoi(s));
+        if (const char *s = std::getenv("LQ_MFMA_DENSE5")) e.mfma_dense5.store(std::atoi(s));
+        if (const char *s = std::getenv("LQ_PRODUCT_FOLD")) e.product_fold.store(std::atoi(s) != 0);
+        if (const char *s = std::getenv("LQ_PLAN_SEARCH")) e.plan_search.store(std::atoi(s) != 0);
         return p;
     }();
     return t;
@@ -276,13 +279,24 @@ StateVector::PassStats StateVector::profile_read(bool reset) {
         stats_.ms += ms;
         stats_.bytes += p.bytes;
         stats_.max_ms = std::max(stats_.max_ms, static_cast<double>(ms));
+        p.rec.ms = ms;
+        p.rec.bytes = p.bytes;
+        pass_log_.push_back(p.rec);
         ev_free_.push_back(p.a);
         ev_free_.push_back(p.b);
     }
     pending_.clear();
     PassStats out = stats_;
-    if (reset) stats_ = PassStats{};
+    if (reset) {
+        stats_ = PassStats{};
+        pass_log_.clear();
+    }
     return out;
+}
+
+std::vector<StateVector::PassRecord> StateVector::profile_passes() {
+    profile_read(false);
+    return pass_log_;
 }
 
 void *StateVector::workspace(int i) {
@@ -415,15 +429,31 @@ void StateVector::synchronize() {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
+namespace {
+bool better_plan(const std::vector<PassPlan> &a, const std::vector<PassPlan> &b);
+}
+
 std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector<void *> &states, int nslots) {
     if (prog.empty()) return std::vector<cd>(static_cast<size_t>(nslots), 0.0);
-    if (prep_pending_ && states.size() == 1 && states[0] == d_) {
+    if (prep_pending_ && states.size() == 1 && states[0] == d_ && tuning().product_fold.load()) {
         // the state is a product state not yet written: its leading
         // one-qubit gates become factors of that product
         std::vector<LOp> rest = prog;
-        if (absorb_into_product(rest, pend_) > 0) {
-            if (rest.empty()) return std::vector<cd>(static_cast<size_t>(nslots), 0.0);
-            return execute(plan(rest, 1), states, nslots);
+        std::vector<cd> f = pend_;
+        if (absorb_into_product(rest, f) > 0) {
+            if (rest.empty()) {
+                pend_.swap(f);
+                return std::vector<cd>(static_cast<size_t>(nslots), 0.0);
+            }
+            Planned folded = plan(rest, 1);
+            if (n_ >= kPlanSearchQubits && tuning().plan_search.load()) {
+                // both forms let the first pass form the state without reading
+                // it; folding removes ops but can cost the greedy planner a pass
+                Planned whole = plan(prog, 1);
+                if (better_plan(whole.passes, folded.passes)) return execute(whole, states, nslots);
+            }
+            pend_.swap(f);
+            return execute(folded, states, nslots);
         }
     }
     return execute(plan(prog, static_cast<int>(states.size())), states, nslots);
@@ -436,7 +466,9 @@ struct PlanCache {
     std::mutex mu;
     std::unordered_map<std::string, std::vector<PassPlan>> map;
     std::deque<std::string> order;
-    static constexpr size_t kMaxEntries = 64;
+    // a serving stream of 10..28-qubit requests at depths 4..16 has ~300
+    // distinct forward and backward program structures
+    static constexpr size_t kMaxEntries = 512;
     static PlanCache &get() {
         static PlanCache c;
         return c;
@@ -459,6 +491,69 @@ std::vector<PassPlan> plan_cached(const std::vector<LOp> &prog, const PlanConfig
     std::vector<PassPlan> passes = plan_passes(prog, cfg);
     std::lock_guard<std::mutex> lk(c.mu);
     // a refused refresh replaces the entry: the angles have moved to the new plan's side
+    if (c.map.insert_or_assign(key, passes).second) {
+        c.order.push_back(std::move(key));
+        if (c.order.size() > PlanCache::kMaxEntries) {
+            c.map.erase(c.order.front());
+            c.order.pop_front();
+        }
+    }
+    return passes;
+}
+
+namespace {
+// Passes first (each reads and writes the states), then LDS transposes.
+bool better_plan(const std::vector<PassPlan> &a, const std::vector<PassPlan> &b) {
+    if (a.size() != b.size()) return a.size() < b.size();
+    auto transposes = [](const std::vector<PassPlan> &v) {
+        size_t t = 0;
+        for (const PassPlan &pp : v) {
+            t += pp.end_transpose ? 1 : 0;
+            for (const TileStage &st : pp.stages) t += st.entry == ENTRY_TRANSPOSE;
+        }
+        return t;
+    };
+    return transposes(a) < transposes(b);
+}
+} // namespace
+
+// The greedy planner's pass count moves irregularly with the per-pass budget
+// (30-qubit HEA, depth 20: 17 to 19 passes over budgets 44..64, and which
+// budget wins changes when the first layer folds into a product state), so
+// for large states, where a pass costs milliseconds, several budgets are
+// planned and the plan with the fewest passes kept; cached like single plans
+// (under the key of budget -1), so each structure searches once.
+std::vector<PassPlan> plan_search(const std::vector<LOp> &prog, const PlanConfig &cfg) {
+    static const double kBudgets[] = {52.0, 44.0, 48.0, 56.0, 60.0, 64.0};
+    auto search = [&] {
+        std::vector<PassPlan> best;
+        bool have = false;
+        for (double b : kBudgets) {
+            PlanConfig c = cfg;
+            c.budget = b;
+            std::vector<PassPlan> p = plan_passes(prog, c);
+            if (!have || better_plan(p, best)) {
+                best.swap(p);
+                have = true;
+            }
+        }
+        return best;
+    };
+    if (!tuning().plan_cache) return search();
+    PlanConfig kc = cfg;
+    kc.budget = -1.0;
+    std::string key = plan_structure_key(prog, kc);
+    PlanCache &c = PlanCache::get();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.map.find(key);
+        if (it != c.map.end()) {
+            std::vector<PassPlan> passes = it->second;
+            if (refresh_plan_coefs(passes, prog)) return passes;
+        }
+    }
+    std::vector<PassPlan> passes = search();
+    std::lock_guard<std::mutex> lk(c.mu);
     if (c.map.insert_or_assign(key, passes).second) {
         c.order.push_back(std::move(key));
         if (c.order.size() > PlanCache::kMaxEntries) {
@@ -498,12 +593,14 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
     const bool use_jit = cfg.reg_rb > 0 && jcfg.enabled && n_ >= jcfg.min_qubits;
     std::vector<PassPlan> passes;
     std::vector<hipFunction_t> fns;
+    std::vector<const std::vector<uint64_t> *> geo;
     auto lookup_all = [&](std::vector<hipFunction_t> &f) {
         size_t missing = 0;
         f.assign(passes.size(), nullptr);
+        geo.assign(passes.size(), nullptr);
         for (size_t p = 0; p < passes.size(); ++p) {
             if (!passes[p].reg) continue;
-            f[p] = jit_lookup_plan(passes[p], n_, dtype_, ns);
+            f[p] = jit_lookup_plan(passes[p], n_, dtype_, ns, &geo[p]);
             missing += f[p] == nullptr;
         }
         return missing;
@@ -524,7 +621,8 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
         scfg.lookahead = t.lookahead;
         std::vector<LOp> split = prog;
         split_1q_rotations(split);
-        passes = plan_cached(split, scfg);
+        passes = (t.budget <= 0 && t.plan_search.load() && n_ >= kPlanSearchQubits) ? plan_search(split, scfg)
+                                                                              : plan_cached(split, scfg);
         const size_t missing = lookup_all(fns);
         const int fallback = t.split_fallback.load();
         if (missing && fallback != 2) {
@@ -541,12 +639,14 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
             if (fallback == 0 || plain_cost < split_cost) {
                 passes.swap(plain);
                 fns.assign(passes.size(), nullptr);
+                geo.assign(passes.size(), nullptr);
             }
         }
     } else {
         passes = plan_cached(prog, cfg);
         if (use_jit) lookup_all(fns);
         else fns.assign(passes.size(), nullptr);
+        geo.resize(passes.size(), nullptr);
     }
     if (t.plan_debug) {
         for (size_t p = 0; p < passes.size(); ++p) {
@@ -562,8 +662,10 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
                          nlds, ntr, pp.end_transpose);
         }
     }
+    geo.resize(passes.size(), nullptr);
     out.passes.swap(passes);
     out.fns.swap(fns);
+    out.geo.swap(geo);
     return out;
 }
 
@@ -577,7 +679,7 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
 
     auto align = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
     std::vector<size_t> op_off(passes.size()), coef_off(passes.size()), part_off(passes.size()),
-        st_off(passes.size());
+        st_off(passes.size()), geo_slots(passes.size(), 0);
     size_t ops_bytes = 0, coef_n = 0, part_n = 0, st_bytes = 0;
     std::vector<SlotRange> ranges(static_cast<size_t>(nslots), SlotRange{0, 0});
     std::vector<char> seen(static_cast<size_t>(nslots), 0);
@@ -587,9 +689,11 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         st_off[p] = st_bytes;
         st_bytes += passes[p].stages.size() * sizeof(TileStage);
         coef_off[p] = coef_n;
-        // + the deferred-preparation marker and the product factors read by
+        // + the deferred-preparation marker, the specialised kernel's table of
+        // bit positions (two per entry) and the product factors read by
         // specialised one-state passes
-        coef_n += passes[p].coef.size() + 1 + 2 * static_cast<size_t>(n_);
+        if (fns[p] && p < pl.geo.size() && pl.geo[p]) geo_slots[p] = (pl.geo[p]->size() + 1) / 2;
+        coef_n += passes[p].coef.size() + 1 + geo_slots[p] + 2 * static_cast<size_t>(n_);
         part_off[p] = part_n;
         const uint64_t ntiles = 1ull << (n_ - static_cast<int>(passes[p].tbits.size()));
         for (size_t s = 0; s < passes[p].slot_ids.size(); ++s) {
@@ -632,8 +736,14 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
             c[i] = make_double2(passes[p].coef[i].real(), passes[p].coef[i].imag());
         const bool form = p == 0 && form_basis;
         c[passes[p].coef.size()] = make_double2(form ? 1.0 : -1.0, 0.0);
+        if (geo_slots[p]) {
+            double2 *g = c + passes[p].coef.size() + 1;
+            g[geo_slots[p] - 1] = make_double2(0.0, 0.0);
+            std::memcpy(g, pl.geo[p]->data(), pl.geo[p]->size() * sizeof(uint64_t));
+        }
+        double2 *f = c + passes[p].coef.size() + 1 + geo_slots[p];
         for (size_t i = 0; i < 2 * static_cast<size_t>(n_); ++i)
-            c[passes[p].coef.size() + 1 + i] = form ? make_double2(pend_[i].real(), pend_[i].imag()) : make_double2(0.0, 0.0);
+            f[i] = form ? make_double2(pend_[i].real(), pend_[i].imag()) : make_double2(0.0, 0.0);
     }
     if (form_basis) prep_pending_ = false;
     if (!ranges.empty()) std::memcpy(hs + ranges_at, ranges.data(), ranges.size() * sizeof(SlotRange));
@@ -683,6 +793,12 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
             // a pass that forms a deferred reset reads nothing of that state
             const int reads = __builtin_popcount(pp.load_mask) - ((p == 0 && form_basis) ? 1 : 0);
             pe.bytes = static_cast<double>(bytes()) * (reads + __builtin_popcount(pp.store_mask));
+            pe.rec.ops = static_cast<uint32_t>(pp.ops.size());
+            pe.rec.stages = static_cast<uint32_t>(pp.stages.size());
+            pe.rec.transposes = pp.end_transpose ? 1u : 0u;
+            for (const TileStage &st : pp.stages) pe.rec.transposes += st.entry == ENTRY_TRANSPOSE;
+            pe.rec.specialised = fns[p] ? 1u : 0u;
+            for (int b : pp.tbits) pe.rec.tile_bits |= 1ull << b;
             hip_check(hipEventRecord(pe.a, stream_), "hipEventRecord");
         }
         if (fns[p])
@@ -764,7 +880,49 @@ void StateVector::apply_matrix(const void *mat, const uint32_t *wires, int nw, c
         launch_stream_copy(tmp, d_, bytes(), stream_);
         return;
     }
+    if (apply_dense5_mfma(m, tb, cm)) return;
     enqueue(lower_matrix(std::move(m), tb, cm, cv, false));
+}
+
+bool StateVector::apply_dense5_mfma(const std::vector<cd> &m, const std::vector<int> &tb, uint64_t cm) {
+    if (dtype_ != 0 || tb.size() != 5 || cm != 0 || n_ < 12 || !tuning().mfma_dense5.load()) return false;
+    int pos[5]; // index bit of local row bit j (tb[0] is the local MSB)
+    uint32_t seen = 0;
+    for (int j = 0; j < 5; ++j) {
+        if (tb[j] < 0 || tb[j] > 4) return false;
+        pos[j] = tb[j];
+        seen |= 1u << tb[j];
+    }
+    if (seen != 31u) return false;
+    // U in the order of the index's low five bits, then its real 64x64 form
+    // [[Re U, -Im U], [Im U, Re U]]
+    auto to_bits = [&](uint32_t r) {
+        uint32_t g = 0;
+        for (int j = 0; j < 5; ++j)
+            if ((r >> (4 - j)) & 1u) g |= 1u << pos[j];
+        return g;
+    };
+    std::vector<float> w(64 * 64);
+    for (uint32_t r = 0; r < 32; ++r)
+        for (uint32_t c = 0; c < 32; ++c) {
+            const cd u = m[r * 32 + c];
+            const uint32_t gr = to_bits(r), gc = to_bits(c);
+            w[gr * 64 + gc] = static_cast<float>(u.real());
+            w[gr * 64 + 32 + gc] = static_cast<float>(-u.imag());
+            w[(32 + gr) * 64 + gc] = static_cast<float>(u.imag());
+            w[(32 + gr) * 64 + 32 + gc] = static_cast<float>(u.real());
+        }
+    flush();
+    void *sp = data();
+    // the previous operator may still be read by a kernel in flight
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    mfma_w_.ensure(w.size() * sizeof(float));
+    hip_check(hipMemcpyAsync(mfma_w_.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, stream_),
+              "hipMemcpyAsync(matrix)");
+    launch_dense5_c64_mfma(sp, sp, static_cast<const float *>(mfma_w_.p), n_, stream_);
+    hip_check(hipGetLastError(), "matrix-core operator launch");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); // w is freed on return
+    return true;
 }
 
 void StateVector::apply_wide(std::vector<cd> m, const std::vector<int> &tb, uint64_t cm, uint64_t cv, const void *in,

@@ -90,6 +90,9 @@ struct OpsList {
 // Dense operators on more targets than this run out of place through a
 // global-memory kernel instead of as tile ops.
 constexpr int kMaxFusedTargets = 10;
+// From this many qubits, specialised programs are planned under several
+// pass budgets and the plan with the fewest passes is kept.
+constexpr int kPlanSearchQubits = 26;
 
 struct EngineTuning {
     int kt_max = 0;      // 0: from the LDS budget
@@ -105,6 +108,20 @@ struct EngineTuning {
     // (its missing passes in the generic kernel)
     // (written through lq_set_split_fallback while other threads plan)
     std::atomic<int> split_fallback{1};
+    // complex64 dense operators on exactly the five lowest index bits (no
+    // controls) run as a 64x64 real GEMM on the matrix cores instead of as a
+    // tile op (lq_set_mfma_dense5 / LQ_MFMA_DENSE5)
+    // (measured on a 30-qubit complex64 state: 3.4 ms per operator against
+    // 167 ms as a tile op; on by default)
+    std::atomic<int> mfma_dense5{1};
+    // Leading one-qubit gates after a reset fold into the product state the
+    // first pass forms, and for large states several pass budgets are planned
+    // (fewest passes kept). Both are correct and tested, and both measured
+    // slower on the 30-qubit HEA (the greedy planner's passes came out
+    // heavier: 124-141 ms against 119.5 ms per circuit), so they are off by
+    // default (LQ_PRODUCT_FOLD=1, LQ_PLAN_SEARCH=1).
+    std::atomic<int> product_fold{0};
+    std::atomic<int> plan_search{0};
 };
 EngineTuning &tuning();
 
@@ -187,6 +204,7 @@ class StateVector {
     struct Planned {
         std::vector<PassPlan> passes;
         std::vector<hipFunction_t> fns; // specialised kernel per pass, or null
+        std::vector<const std::vector<uint64_t> *> geo; // its bit-position table (null or empty: none)
         int ns = 0;
     };
     Planned plan(const std::vector<LOp> &prog, int ns);
@@ -210,6 +228,13 @@ class StateVector {
 
     void profile(bool on);
     PassStats profile_read(bool reset);
+    // one record per profiled pass since the last profile_read(reset=true)
+    struct PassRecord {
+        double ms = 0.0, bytes = 0.0;
+        uint32_t ops = 0, stages = 0, transposes = 0, specialised = 0;
+        uint64_t tile_bits = 0; // physical bits of the pass's tile
+    };
+    std::vector<PassRecord> profile_passes();
     // passes launched as specialised kernels / generic tile kernels
     void pass_counts(bool reset, uint64_t *specialised, uint64_t *generic);
 
@@ -239,7 +264,9 @@ class StateVector {
     struct Pending {
         hipEvent_t a, b;
         double bytes;
+        PassRecord rec;
     };
+    std::vector<PassRecord> pass_log_;
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> ev_free_;
     PassStats stats_;
@@ -256,6 +283,8 @@ class StateVector {
     // CSR operator on the device: offsets, then column indices, then values
     DevBuf sparse_;
     DevBuf wide_; // matrix of the last operator wider than a tile op
+    DevBuf mfma_w_; // real 64x64 form of the last matrix-core operator
+    bool apply_dense5_mfma(const std::vector<cd> &m, const std::vector<int> &tb, uint64_t cm);
     size_t csr_indices_off_ = 0, csr_values_off_ = 0;
     std::vector<char> sparse_host_;
     CsrView sparse_view(const Obs &o);

@@ -161,8 +161,8 @@ struct Mapping {
 
 class Gen {
   public:
-    Gen(const PassPlan &p, int n, int dtype, int ns)
-        : p_(p), dtype_(dtype), ns_(ns), nt_(jit_config().nontemporal) {
+    Gen(const PassPlan &p, int n, int dtype, int ns, bool lift)
+        : p_(p), dtype_(dtype), ns_(ns), nt_(jit_config().nontemporal), lift_(lift) {
         kt_ = static_cast<int>(p.tbits.size());
         rb_ = kt_ - 8;
         E_ = 1 << rb_;
@@ -174,13 +174,23 @@ class Gen {
         LQ_ABORT_IF(per_state > 65536, "kernel generator: tile too large");
     }
 
+    std::vector<uint64_t> &geo() { return geo_; }
+
     std::string run() {
         std::ostringstream &o = o_;
+        o << "#define LQ_GEO " << p_.coef.size() + 1 << "\n";
         o << "#define LQ_REAL " << (dtype_ ? "double" : "float") << "\n";
         o << "#define LQ_CPLX " << (dtype_ ? "double2" : "float2") << "\n";
         o << "#define LQ_SWZ " << (dtype_ ? "(l ^ (((l >> 3) ^ (l >> 6) ^ (l >> 9)) & 7u))" : "(l ^ (((l >> 4) ^ (l >> 8)) & 15u))")
           << "\n";
         o << kPrelude;
+        if (lift_) {
+            // positions of the pass's bits in the state index are read at run
+            // time from the table after the coefficients, so passes of the
+            // same shape share one kernel whatever the qubit count
+            o << "typedef __attribute__((address_space(4))) const unsigned long long kgeo;\n";
+            o << "#define GQ(i) (((const kgeo *)(coef + LQ_GEO))[i])\n";
+        }
         o << "// lds_bytes " << jit_lds_bytes(kt_, dtype_, ns_) << "\n";
         o << "extern \"C\" __global__ __launch_bounds__(256) ";
         if (const int w = jit_waves_per_eu(dtype_, ns_); w > 0) o << "__attribute__((amdgpu_waves_per_eu(" << w << "))) ";
@@ -220,7 +230,7 @@ class Gen {
             // input from the factors that follow instead of reading it
             o << "  const double2 bmark = ((const kcoef *)coef)[" << p_.coef.size() << "];\n";
             o << "  if (bmark.x > 0.0) {\n";
-            emit_product(lm, p_.coef.size() + 1);
+            emit_product(lm);
             o << "  } else {\n";
             emit_loads("v", "gl0", lm);
             o << "  }\n";
@@ -230,27 +240,70 @@ class Gen {
         o << "  }\n";
         emit_tile_body(m0, nslots, lm);
         o << "}\n";
-        return o.str();
+        // the product factors follow the position table
+        return "#define LQ_FOFF " + std::to_string(p_.coef.size() + 1 + (geo_.size() + 1) / 2) + "\n" + o.str();
     }
 
   private:
     const PassPlan &p_;
     int dtype_, ns_;
     bool nt_;
+    bool lift_;                // positions in the index read at run time (GQ table)
+    std::vector<uint64_t> geo_; // that table, in order of use
+
+    // a 64-bit constant that depends on where the pass's bits sit in the index
+    std::string K(uint64_t v) {
+        if (!lift_) return hex64(v);
+        geo_.push_back(v);
+        return "GQ(" + std::to_string(geo_.size() - 1) + ")";
+    }
+    // a bit position of the index (a shift amount)
+    std::string Sh(int v) {
+        if (!lift_) return std::to_string(v);
+        geo_.push_back(static_cast<uint64_t>(v));
+        return "(unsigned)GQ(" + std::to_string(geo_.size() - 1) + ")";
+    }
+    // the tile's base index: the tile number with zeros at the tile bits
+    std::string base_expr(const char *tile) {
+        if (!lift_) return deposit(tile, geom_.ob, geom_.nout, true);
+        std::ostringstream e;
+        e << "[&]() { unsigned long long x = (unsigned long long)" << tile << " << " << geom_.c << ";";
+        for (int j = geom_.c; j < kt_; ++j) {
+            e << " { const unsigned s = " << Sh(geom_.tb[j]) << "; x = ((x >> s) << (s + 1u)) | (x & ((1ull << s) - 1ull)); }";
+        }
+        e << " return x; }()";
+        return e.str();
+    }
+    // bits of tid (bit j -> index position pos[j])
+    std::string tid_expr(const int *pos) {
+        if (!lift_) return deposit("tid", pos, 8, true);
+        std::ostringstream e;
+        bool any = false;
+        for (int j = 0; j < 8;) {
+            int len = 1;
+            while (j + len < 8 && pos[j + len] == pos[j] + len) ++len;
+            if (any) e << " | ";
+            e << "((((unsigned long long)tid >> " << j << ") & " << ((1u << len) - 1u) << "ull) << " << Sh(pos[j]) << ")";
+            any = true;
+            j += len;
+        }
+        return e.str();
+    }
+    std::string fac(const std::string &i) { return "((const kcoef *)coef)[LQ_FOFF + " + i + "]"; }
 
     void emit_base_gl(const char *tile, const char *gl, const Mapping &m) {
         o_ << "  const unsigned long long " << (std::string(gl) == "gl0" ? "b0" : "bn") << " = "
-           << deposit(tile, geom_.ob, geom_.nout, true) << ";\n";
+           << base_expr(tile) << ";\n";
         int pos[8];
         for (int j = 0; j < 8; ++j) pos[j] = geom_.tb[m.h[j]];
         o_ << "  const unsigned long long " << gl << "_ = " << (std::string(gl) == "gl0" ? "b0" : "bn") << " | "
-           << deposit("tid", pos, 8, true) << ";\n";
+           << tid_expr(pos) << ";\n";
     }
     // v0 = the product state prod_q f_q[bit q] over the tile, f_q[b] at
-    // coefficient F + 2q + b: the factors of the bits fixed per thread
-    // (outside and thread bits) multiply into one per-thread value, the
-    // register bits' factors into a 2^rb table (FP64 in both precisions)
-    void emit_product(const Mapping &m, size_t F) {
+    // factor 2q + b: the factors of the bits fixed per thread (outside and
+    // thread bits) multiply into one per-thread value, the register bits'
+    // factors into a 2^rb table (FP64 in both precisions)
+    void emit_product(const Mapping &m) {
         std::ostringstream &o = o_;
         int rg[kMaxRegBits];
         uint64_t regs = 0;
@@ -258,17 +311,34 @@ class Gen {
             rg[i] = geom_.tb[m.r[i]];
             regs |= 1ull << rg[i];
         }
-        auto fac = [&](int q, int b) {
-            return "((const kcoef *)coef)[" + std::to_string(F + 2 * static_cast<size_t>(q) + static_cast<size_t>(b)) + "]";
-        };
         o << "    double2 pw[" << E_ << "];\n";
         o << "    pw[0] = make_double2(1.0, 0.0);\n";
-        for (int q = 0; q < geom_.n; ++q) {
-            if ((regs >> q) & 1ull) continue;
-            o << "    pw[0] = zm(pw[0], ((gl0_ >> " << q << ") & 1ull) ? " << fac(q, 1) << " : " << fac(q, 0) << ");\n";
+        if (lift_) {
+            // the bits other than the register bits: a count, then up to 40
+            // positions (a fixed block, so the table's layout does not
+            // depend on the qubit count)
+            const size_t at = geo_.size();
+            geo_.push_back(0);
+            for (int q = 0; q < geom_.n; ++q) {
+                if ((regs >> q) & 1ull) continue;
+                geo_.push_back(static_cast<uint64_t>(q));
+            }
+            geo_[at] = geo_.size() - at - 1;
+            geo_.resize(at + 41, 0);
+            o << "    for (unsigned i = 0; i < (unsigned)GQ(" << at << "); ++i) {\n";
+            o << "      const unsigned q = (unsigned)GQ(" << at + 1 << " + i);\n";
+            o << "      pw[0] = zm(pw[0], ((gl0_ >> q) & 1ull) ? " << fac("2u * q + 1u") << " : " << fac("2u * q") << ");\n";
+            o << "    }\n";
+        } else {
+            for (int q = 0; q < geom_.n; ++q) {
+                if ((regs >> q) & 1ull) continue;
+                o << "    pw[0] = zm(pw[0], ((gl0_ >> " << q << ") & 1ull) ? " << fac(std::to_string(2 * q + 1)) << " : "
+                  << fac(std::to_string(2 * q)) << ");\n";
+            }
         }
         for (int i = 0; i < rb_; ++i) {
-            o << "    { const double2 f0 = " << fac(rg[i], 0) << ", f1 = " << fac(rg[i], 1) << ";\n";
+            const std::string q = Sh(rg[i]);
+            o << "    { const double2 f0 = " << fac("2u * " + q) << ", f1 = " << fac("2u * " + q + " + 1u") << ";\n";
             for (int e = (1 << i) - 1; e >= 0; --e) {
                 o << "      pw[" << (e | (1 << i)) << "] = zm(pw[" << e << "], f1);\n";
                 o << "      pw[" << e << "] = zm(pw[" << e << "], f0);\n";
@@ -281,7 +351,7 @@ class Gen {
         for (int s = 0; s < ns_; ++s) {
             if (!((p_.load_mask >> s) & 1u)) continue;
             for (int e = 0; e < E_; ++e) {
-                const std::string addr = std::string("&sv") + std::to_string(s) + "[" + gl + "_ | " + hex64(elem_global(m, e)) + "]";
+                const std::string addr = std::string("&sv") + std::to_string(s) + "[" + gl + "_ | " + K(elem_global(m, e)) + "]";
                 if (nt_)
                     o_ << "  " << dst << s << "[" << e << "] = ldnt(" << addr << ");\n";
                 else
@@ -318,7 +388,7 @@ class Gen {
     void emit_tile_body(const Mapping &m0, int nslots, const Mapping &lm) {
         std::ostringstream &o = o_;
         o << "  {\n";
-        o << "  const unsigned long long base = " << deposit("tile", geom_.ob, geom_.nout, true) << ";\n";
+        o << "  const unsigned long long base = " << base_expr("tile") << ";\n";
         o << "  const unsigned cz = opaque0();\n";
         if (nslots) {
             o << "  for (unsigned i = tid; i < 128u; i += 256u) dred[i >> 5][i & 31u] = make_double2(0.0, 0.0);\n";
@@ -352,9 +422,9 @@ class Gen {
                 if (!((p_.store_mask >> s) & 1u)) continue;
                 for (int e = 0; e < E_; ++e) {
                     if (nt_)
-                        o << "  stnt(&sv" << s << "[" << gs << " | " << hex64(elem_global(cur_, e)) << "], v" << s << "[" << e << "]);\n";
+                        o << "  stnt(&sv" << s << "[" << gs << " | " << K(elem_global(cur_, e)) << "], v" << s << "[" << e << "]);\n";
                     else
-                        o << "  sv" << s << "[" << gs << " | " << hex64(elem_global(cur_, e)) << "] = v" << s << "[" << e << "];\n";
+                        o << "  sv" << s << "[" << gs << " | " << K(elem_global(cur_, e)) << "] = v" << s << "[" << e << "];\n";
                 }
             }
         }
@@ -402,7 +472,7 @@ class Gen {
     void emit_gl(const char *name, const Mapping &m) {
         int pos[8];
         for (int j = 0; j < 8; ++j) pos[j] = geom_.tb[m.h[j]];
-        o_ << "  const unsigned long long " << name << " = base | " << deposit("tid", pos, 8, true) << ";\n";
+        o_ << "  const unsigned long long " << name << " = base | " << tid_expr(pos) << ";\n";
     }
     void transpose(const Mapping &to) {
         std::ostringstream &o = o_;
@@ -536,10 +606,11 @@ class Gen {
         return true;
     }
     // returns the guard expression for the thread part ("" when none)
-    std::string thread_guard(uint64_t cm, uint64_t cv) const {
+    std::string thread_guard(uint64_t cm, uint64_t cv) {
         const uint64_t tm = cm & ~regmask();
         if (!tm) return "";
-        return "((gt & " + hex64(tm) + ") == " + hex64(cv & tm) + ")";
+        const std::string a = K(tm);
+        return "((gt & " + a + ") == " + K(cv & tm) + ")";
     }
 
     std::vector<int> states_of(uint32_t smask) const {
@@ -646,7 +717,7 @@ class Gen {
         case TOP_TPHASE: {
             // product of one-bit factors on bits fixed per thread, then one multiply per amplitude
             for (uint32_t j = 0; j < t.k; ++j) {
-                const std::string pick = "(((gt >> " + std::to_string(t.gpos[j]) + ") & 1ull) ? ldc(coef, cz + " +
+                const std::string pick = "(((gt >> " + Sh(t.gpos[j]) + ") & 1ull) ? ldc(coef, cz + " +
                                          std::to_string(off + 2 * j + 1) + ") : ldc(coef, cz + " +
                                          std::to_string(off + 2 * j) + "))";
                 if (j == 0) o << "    cplx f = " << pick << ";\n";
@@ -680,7 +751,7 @@ class Gen {
             const uint64_t tm = t.gpmask & ~regmask();
             o << "    const cplx d0 = ldc(coef, cz + " << off << "), d1_ = ldc(coef, cz + " << off + 1 << ");\n";
             if (tm) {
-                o << "    const bool pt = (pc(gt & " << hex64(tm) << ") & 1u) != 0u;\n";
+                o << "    const bool pt = (pc(gt & " << K(tm) << ") & 1u) != 0u;\n";
                 o << "    const cplx q0 = pt ? d1_ : d0, q1 = pt ? d0 : d1_;\n";
             } else {
                 o << "    const cplx q0 = d0, q1 = d1_;\n";
@@ -701,7 +772,7 @@ class Gen {
                 reg_of[j] = reg_index_of_global(t.gpos[j]);
                 if (reg_of[j] < 0) {
                     if (any) ti << " | ";
-                    ti << "((unsigned)((gt >> " << t.gpos[j] << ") & 1ull) << " << (k - 1 - j) << ")";
+                    ti << "((unsigned)((gt >> " << Sh(t.gpos[j]) << ") & 1ull) << " << (k - 1 - j) << ")";
                     any = true;
                 }
             }
@@ -770,10 +841,13 @@ class Gen {
             if (ttm) tt.emplace_back(d, ttm);
         }
         o << "    const unsigned p0 = (0u";
-        for (const auto &x : tt) o << " + pc(gt & (gt >> " << x.first << ") & " << hex64(x.second) << ")";
+        for (const auto &x : tt) {
+            const std::string d = Sh(static_cast<int>(x.first));
+            o << " + pc(gt & (gt >> " << d << ") & " << K(x.second) << ")";
+        }
         o << ") & 1u;\n";
         for (int i = 0; i < rb_; ++i)
-            if (tmask[i]) o << "    const unsigned l" << i << " = pc(gt & " << hex64(tmask[i]) << ") & 1u;\n";
+            if (tmask[i]) o << "    const unsigned l" << i << " = pc(gt & " << K(tmask[i]) << ") & 1u;\n";
         for (int e = 0; e < E_; ++e) {
             int q = 0;
             for (uint32_t pr : qpairs)
@@ -790,7 +864,7 @@ class Gen {
 };
 } // namespace
 
-std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns) {
+std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns, std::vector<uint64_t> *geo) {
     if (!p.reg || p.stages.empty()) return "";
     const int kt = static_cast<int>(p.tbits.size());
     if (kt < 11 || kt > 13 || ns > 2 || p.slot_ids.size() > 32) return "";
@@ -815,8 +889,10 @@ std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns) {
         }
         if (t.flags & TF_ZERO_UNCTRL) return "";
     }
-    Gen g(p, n, dtype, ns);
-    return g.run();
+    Gen g(p, n, dtype, ns, geo != nullptr);
+    std::string src = g.run();
+    if (geo) geo->swap(g.geo());
+    return src;
 }
 
 // ---------------------------------------------------------------------------
@@ -835,6 +911,7 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_SCALE")) j.scale_rotations = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_WAVES")) j.waves_per_eu = std::atoi(s);
         if (const char *s = std::getenv("LQ_JIT_DIRECT")) j.direct_io = std::atoi(s);
+        if (const char *s = std::getenv("LQ_JIT_LIFT")) j.lift = std::atoi(s) != 0;
         return j;
     }();
     return c;
@@ -877,25 +954,25 @@ class Service {
 
     // A structural key of the pass stands for its source, so a pass seen
     // before needs no source generation.
-    hipFunction_t lookup_key(const std::string &key, const PassPlan &p, int n, int dtype, int ns) {
-        std::shared_ptr<JEntry> hit;
-        bool known = false;
+    hipFunction_t lookup_key(const std::string &key, const PassPlan &p, int n, int dtype, int ns,
+                             const std::vector<uint64_t> **geo) {
+        const KeyEntry *hit = nullptr;
         {
             std::lock_guard<std::mutex> lk(mu_);
             auto it = keys_.find(key);
-            if (it != keys_.end()) {
-                known = true;
-                hit = it->second;
-            }
+            if (it != keys_.end()) hit = &it->second;
         }
-        if (known) return hit ? resolve(hit) : nullptr;
-        const std::string src = jit_pass_source(p, n, dtype, ns);
-        std::shared_ptr<JEntry> e = src.empty() ? nullptr : enqueue(src);
-        {
+        if (!hit) {
+            KeyEntry k;
+            std::vector<uint64_t> g;
+            const std::string src = jit_pass_source(p, n, dtype, ns, jit_config().lift ? &g : nullptr);
+            k.e = src.empty() ? nullptr : enqueue(src);
+            k.geo = std::move(g);
             std::lock_guard<std::mutex> lk(mu_);
-            keys_.emplace(key, e);
+            hit = &keys_.emplace(key, std::move(k)).first->second; // entries are never erased
         }
-        return e ? resolve(e) : nullptr;
+        if (geo) *geo = &hit->geo;
+        return hit->e ? resolve(hit->e) : nullptr;
     }
 
     void wait() {
@@ -925,7 +1002,11 @@ class Service {
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
     std::unordered_map<std::string, std::shared_ptr<JEntry>> map_;
-    std::unordered_map<std::string, std::shared_ptr<JEntry>> keys_; // null: not specialisable
+    struct KeyEntry {
+        std::shared_ptr<JEntry> e; // null: not specialisable
+        std::vector<uint64_t> geo; // the pass's position table
+    };
+    std::unordered_map<std::string, KeyEntry> keys_;
 
     // Entry for `src`, queueing its compile (or compiling now in sync mode)
     // the first time; null after stop().
@@ -1052,7 +1133,7 @@ class Service {
 
 hipFunction_t jit_lookup(const std::string &src) { return Service::get().lookup(src); }
 
-hipFunction_t jit_lookup_plan(const PassPlan &p, int n, int dtype, int ns) {
+hipFunction_t jit_lookup_plan(const PassPlan &p, int n, int dtype, int ns, const std::vector<uint64_t> **geo) {
     // every field the generated source depends on (ops and stages are
     // zero-initialised PODs, so their bytes are a faithful key)
     std::string key;
@@ -1065,7 +1146,7 @@ hipFunction_t jit_lookup_plan(const PassPlan &p, int n, int dtype, int ns) {
     key.append(reinterpret_cast<const char *>(p.tbits.data()), p.tbits.size() * sizeof(int));
     key.append(reinterpret_cast<const char *>(p.ops.data()), p.ops.size() * sizeof(TileOp));
     key.append(reinterpret_cast<const char *>(p.stages.data()), p.stages.size() * sizeof(TileStage));
-    return Service::get().lookup_key(key, p, n, dtype, ns);
+    return Service::get().lookup_key(key, p, n, dtype, ns, geo);
 }
 void jit_wait() { Service::get().wait(); }
 void jit_shutdown() { Service::get().stop(); }

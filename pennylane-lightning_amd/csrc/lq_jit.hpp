@@ -17,7 +17,12 @@ namespace lqamd {
 
 // Source for a register pass of n qubits, dtype 0/1 (complex64/128), ns
 // states; empty when the pass has stages the generator does not cover.
-std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns);
+// With `geo`, the bit positions of the pass in the state index are not
+// written into the source: the kernel reads them from a table placed right
+// after the pass's coefficients and the deferred-preparation entry (the
+// values are returned in *geo, in table order), so passes of the same shape
+// at other positions or qubit counts share the kernel.
+std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns, std::vector<uint64_t> *geo = nullptr);
 
 struct JitConfig {
     bool enabled = true;
@@ -31,6 +36,10 @@ struct JitConfig {
                               // so that more workgroups fit per CU
     int waves_per_eu = -1;    // > 0: ask the compiler to fit that many waves per SIMD; 0: no
                               // request; -1: by kind (one-state passes: 3 complex128, 4 complex64)
+    bool lift = false;        // bit positions read at run time, so passes of one shape at other
+                              // positions or qubit counts share a kernel (LQ_JIT_LIFT=1). Off:
+                              // on the serving stream's 5315 passes (n 17..28, depth 4..16) it
+                              // cuts distinct kernels only from 3104 to 2331
     int direct_io = 1;        // loads/stores in the first/last stage's mapping (no entry/exit
                               // transpose): 0 never, 1 where measured to pay, 2 whenever they coalesce
 };
@@ -45,7 +54,10 @@ size_t jit_lds_bytes(int kt, int dtype, int ns);
 hipFunction_t jit_lookup(const std::string &src);
 // Same for a planned pass, found by its structure without generating the
 // source again once the pass has been seen.
-hipFunction_t jit_lookup_plan(const PassPlan &p, int n, int dtype, int ns);
+// *geo (may be null) receives the pass's position table (empty when the
+// kernel has the positions in its source); it stays valid for the process.
+hipFunction_t jit_lookup_plan(const PassPlan &p, int n, int dtype, int ns,
+                              const std::vector<uint64_t> **geo = nullptr);
 // Block until every queued compile has finished.
 void jit_wait();
 // Drop queued compiles and wait for those in flight; no new compiles start

@@ -104,6 +104,10 @@ void launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t 
 template <typename T>
 void launch_dense_wide(const void *in, void *out, const double2 *M, const int *bits, int k, uint64_t cmask,
                        uint64_t cval, int n, hipStream_t s);
+// complex64, in place allowed: out = U in for a dense 32x32 operator on index
+// bits 0..4 (bit 4 the local MSB), given as the row-major real 64x64 matrix
+// W = [[Re U, -Im U], [Im U, Re U]] on the device; matrix cores; n >= 12
+void launch_dense5_c64_mfma(const void *in, void *out, const float *W, int n, hipStream_t s);
 template <typename T> void launch_scale(void *a, int n, double re, double im, hipStream_t s);
 // zero amplitudes whose bit `bit` != branch, then scale the rest by `factor`
 template <typename T>

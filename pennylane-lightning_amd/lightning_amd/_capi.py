@@ -83,11 +83,14 @@ _SIGS = {
     "lq_sv_overlap": (C.c_int, [_P, _P, _DP, _DP]),
     "lq_sv_profile": (C.c_int, [_P, C.c_int]),
     "lq_sv_profile_read": (C.c_int, [_P, C.c_int, _U64P, _DP, _DP, _DP]),
+    "lq_sv_profile_passes": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_uint64)]),
     "lq_sv_stream": (C.c_int, [_P, C.POINTER(_P)]),
     "lq_jit_wait": (C.c_int, []),
     "lq_jit_shutdown": (C.c_int, []),
     "lq_jit_stats": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lq_set_split_fallback": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
+    "lq_set_mfma_dense5": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
+    "lq_set_engine_option": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
     "lq_sv_pass_counts": (C.c_int, [_P, C.c_int, _U64P, _U64P]),
 }
 

@@ -74,6 +74,23 @@ def set_split_fallback(mode: int) -> int:
     return prev.value
 
 
+def set_mfma_dense5(on: bool) -> bool:
+    """Run complex64 dense operators on exactly the five lowest index bits
+    (wires n-5 .. n-1, no controls) on the matrix cores. Returns the previous
+    setting."""
+    prev = C.c_int()
+    check(lib.lq_set_mfma_dense5(1 if on else 0, C.byref(prev)))
+    return bool(prev.value)
+
+
+def set_engine_option(name: str, value: int) -> int:
+    """Set a process-wide engine option ("split_fallback", "mfma_dense5",
+    "product_fold", "plan_search"); returns the previous value."""
+    prev = C.c_int()
+    check(lib.lq_set_engine_option(name.encode(), int(value), C.byref(prev)))
+    return prev.value
+
+
 def jit_stats() -> dict:
     """Specialised kernels loaded, compiles failed, compiles pending."""
     r, f, p = C.c_int(), C.c_int(), C.c_int()
@@ -167,6 +184,22 @@ class _StateVectorBase:
         p, ms, b, mx = C.c_uint64(), C.c_double(), C.c_double(), C.c_double()
         check(lib.lq_sv_profile_read(self._h, int(bool(reset)), C.byref(p), C.byref(ms), C.byref(b), C.byref(mx)))
         return {"passes": p.value, "ms": ms.value, "bytes": b.value, "max_ms": mx.value}
+
+    def profile_passes(self) -> list:
+        """Per-pass records of the profiled passes since the last reset:
+        kernel ms, algorithmic bytes, ops, register stages, LDS transposes,
+        specialised (bool), tile bits."""
+        cnt = C.c_uint64()
+        check(lib.lq_sv_profile_passes(self._h, None, 0, C.byref(cnt)))
+        buf = (C.c_double * (7 * max(cnt.value, 1)))()
+        check(lib.lq_sv_profile_passes(self._h, buf, cnt.value, C.byref(cnt)))
+        out = []
+        for i in range(cnt.value):
+            r = buf[7 * i:7 * i + 7]
+            tb = int(r[6])
+            out.append({"ms": r[0], "bytes": r[1], "ops": int(r[2]), "stages": int(r[3]), "transposes": int(r[4]),
+                        "specialised": bool(r[5]), "tile_bits": [b for b in range(64) if (tb >> b) & 1]})
+        return out
 
     def stream_ptr(self) -> int:
         s = C.c_void_p()

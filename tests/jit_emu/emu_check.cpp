@@ -16,6 +16,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <random>
 #include <sstream>
@@ -87,6 +88,7 @@ static std::string transform(std::string s) {
     rep("__shared__ double2 dred[4][32];", "static double2 dred[4][32];");
     rep("asm volatile(\"\" : \"+s\"(z));", "");
     rep("typedef __attribute__((address_space(4))) const double2 kcoef;", "typedef const double2 kcoef;");
+    rep("typedef __attribute__((address_space(4))) const unsigned long long kgeo;", "typedef const unsigned long long kgeo;");
     rep("typedef real nvec __attribute__((ext_vector_type(2)));", "struct nvec { real x, y; };");
     return "#include \"emu_rt.hpp\"\n" + s;
 }
@@ -192,7 +194,10 @@ int main(int argc, char **argv) {
             else { float *p = (float *)buf[s].data(); p[2 * i] = (float)v.real(); p[2 * i + 1] = (float)v.imag(); }
         }
     for (size_t p = 0; p < passes.size(); ++p) {
-        const std::string src = jit_pass_source(passes[p], n, dtype, ns);
+        // EMU_LIFT=1: bit positions read from the table after the coefficients
+        const bool lift = getenv("EMU_LIFT") && atoi(getenv("EMU_LIFT")) != 0;
+        std::vector<uint64_t> geo;
+        const std::string src = jit_pass_source(passes[p], n, dtype, ns, lift ? &geo : nullptr);
         if (src.empty()) { printf("pass %zu not specialisable\n", p); return 2; }
         const std::string base = work + "/emu_pass_" + std::to_string(which) + "_" + std::to_string(dtype) + "_" + std::to_string(p);
         { std::ofstream f(base + ".cpp"); f << transform(src); }
@@ -202,10 +207,13 @@ int main(int argc, char **argv) {
         void *h = dlopen((base + ".so").c_str(), RTLD_NOW | RTLD_LOCAL);
         if (!h) { printf("dlopen: %s\n", dlerror()); return 4; }
         Launch L = (Launch)dlsym(h, "emu_launch");
-        // coefficients, then the deferred-preparation marker and the product
-        // factors, as the engine writes them
+        // coefficients, then the deferred-preparation marker, the bit-position
+        // table (two entries per slot) and the product factors, as the engine
+        // writes them
         const size_t nc = passes[p].coef.size();
-        std::vector<double2> coef(nc + 1 + 2 * static_cast<size_t>(n), double2{0.0, 0.0});
+        const size_t gs = (geo.size() + 1) / 2;
+        std::vector<double2> coef(nc + 1 + gs + 2 * static_cast<size_t>(n), double2{0.0, 0.0});
+        if (gs) std::memcpy(&coef[nc + 1], geo.data(), geo.size() * sizeof(uint64_t));
         for (size_t i = 0; i < nc; ++i) coef[i] = {passes[p].coef[i].real(), passes[p].coef[i].imag()};
         const bool form = p == 0 && basis >= 0;
         if (form && !((passes[p].load_mask & 1u) && (passes[p].store_mask & 1u))) {
@@ -214,7 +222,7 @@ int main(int argc, char **argv) {
         }
         coef[nc] = form ? double2{1.0, 0.0} : double2{-1.0, 0.0};
         if (form)
-            for (size_t i = 0; i < fac.size(); ++i) coef[nc + 1 + i] = {fac[i].real(), fac[i].imag()};
+            for (size_t i = 0; i < fac.size(); ++i) coef[nc + 1 + gs + i] = {fac[i].real(), fac[i].imag()};
         const uint64_t ntiles = 1ull << (n - (int)passes[p].tbits.size());
         std::vector<double2> part(std::max<size_t>(1, passes[p].slot_ids.size() * ntiles), {0, 0});
         const unsigned grid = (unsigned)ntiles;

@@ -25,6 +25,14 @@ SV = {128: L.StateVectorC128, 64: L.StateVectorC64}
 TOL = {128: TOL_C128, 64: TOL_C64}
 
 
+@pytest.fixture(autouse=True, scope="module")
+def product_fold_on():
+    """Folding is an engine option (off by default): these tests turn it on."""
+    prev = L.set_engine_option("product_fold", 1)
+    yield
+    L.set_engine_option("product_fold", prev)
+
+
 def one_qubit_layer(n, seed):
     rng = np.random.default_rng(seed)
     names = ["RX", "RY", "RZ", "Rot", "Hadamard", "S", "T", "SX", "PauliY", "PhaseShift"]
