@@ -110,6 +110,22 @@ int lq_sv_collapse(lq_sv_t sv, uint32_t wire, int branch);
 /* ---- Measurements (lq/measurements/MeasurementsLQubit.hpp) ---------------- */
 /* probs() when all != 0, else probs(wires) (:90-163); out has 2^n or 2^nw doubles */
 int lq_probs(lq_sv_t sv, const uint32_t *wires, uint32_t nw, int all, double *out);
+/* No counterpart in the reference. A variant of lq_probs(wires) for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops): the state is the caller's device memory (2^num_qubits
+ * amplitudes, complex64 for dtype LQ_C64, complex128 for LQ_C128), the
+ * result 2^nw doubles in the caller's device memory out_dev (wires[0] the
+ * most significant bit of the result index, as lq_probs), enqueued on
+ * `stream` (a hipStream_t; NULL: the null stream); the caller synchronizes.
+ * No allocation: the per-block partial histograms go to a fixed device array
+ * of 512 x 2^10 doubles (4 MiB, held by each GPU the library runs on until
+ * the process ends), so nw is limited to 1..10; every call on a GPU shares
+ * that array, and calls that overlap on one GPU race. lq_probs_noalloc_init
+ * takes the array's address on the current device (optional: the first call
+ * on a device does it otherwise). */
+int lq_probs_noalloc_init(void);
+int lq_probs_noalloc(const void *state_dev, uint32_t num_qubits, int dtype, const uint32_t *wires, uint32_t nw,
+                     double *out_dev, void *stream);
 /* expval(name, wires) for Identity/PauliX/PauliY/PauliZ/Hadamard (:247-305) */
 int lq_expval_named(lq_sv_t sv, const char *name, const uint32_t *wires, uint32_t nw, double *out);
 /* several one-wire named expvals in one read of the state */

@@ -656,6 +656,18 @@ int lq_set_engine_option(const char *name, int value, int *previous) {
     });
 }
 
+int lq_probs_noalloc_init(void) {
+    return guard([&] { probs_noalloc_init(); });
+}
+
+int lq_probs_noalloc(const void *state_dev, uint32_t num_qubits, int dtype, const uint32_t *wires, uint32_t nw,
+                     double *out_dev, void *stream) {
+    return guard([&] {
+        probs_noalloc(state_dev, static_cast<int>(num_qubits), dtype, wires, static_cast<int>(nw), out_dev,
+                      static_cast<hipStream_t>(stream));
+    });
+}
+
 int lq_set_mfma_dense5(int on, int *previous) {
     return guard([&] {
         const int old = tuning().mfma_dense5.exchange(on ? 1 : 0);

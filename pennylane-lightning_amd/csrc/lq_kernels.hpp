@@ -108,6 +108,14 @@ void launch_dense_wide(const void *in, void *out, const double2 *M, const int *b
 // bits 0..4 (bit 4 the local MSB), given as the row-major real 64x64 matrix
 // W = [[Re U, -Im U], [Im U, Re U]] on the device; matrix cores; n >= 12
 void launch_dense5_c64_mfma(const void *in, void *out, const float *W, int n, hipStream_t s);
+// Probabilities of 1..kNoAllocProbsMaxWires wires of a caller-owned device
+// state into device memory `out` (2^nw doubles) on stream s, with the
+// partial histograms in a fixed device array (no allocation; calls that
+// overlap on one GPU share it). probs_noalloc_init takes the array's address
+// on the current device (the first probs_noalloc call does it otherwise).
+constexpr int kNoAllocProbsMaxWires = 10;
+void probs_noalloc_init();
+void probs_noalloc(const void *state, int n, int dtype, const uint32_t *wires, int nw, double *out, hipStream_t s);
 template <typename T> void launch_scale(void *a, int n, double re, double im, hipStream_t s);
 // zero amplitudes whose bit `bit` != branch, then scale the rest by `factor`
 template <typename T>

@@ -91,6 +91,9 @@ _SIGS = {
     "lq_set_split_fallback": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     "lq_set_mfma_dense5": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     "lq_set_engine_option": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+    "lq_probs_noalloc_init": (C.c_int, []),
+    "lq_probs_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.c_uint32,
+                                   C.c_void_p, C.c_void_p]),
     "lq_sv_pass_counts": (C.c_int, [_P, C.c_int, _U64P, _U64P]),
 }
 
