@@ -233,7 +233,7 @@ typedef struct {
     int dtype;               /* LQ_C64 / LQ_C128 */
     int batched;             /* 1: continuous batching; 0: plain FIFO */
     uint64_t hbm_budget;     /* bytes; 0: 250 GiB */
-    int small_max_qubits;    /* -1: the packed kernel's limit */
+    int small_max_qubits;    /* -1: sizes whose states fit the packed kernel's LDS (12 qubits complex128, 13 complex64); at most 16 */
     int max_batch;           /* requests per packed launch (0: 4096) */
     int batch_streams;       /* packed launches in flight (0: 2) */
     int large_workers;       /* concurrent engine requests (0: 4) */

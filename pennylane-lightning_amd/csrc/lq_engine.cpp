@@ -68,6 +68,7 @@ EngineTuning &tuning() {
         if (const char *s = std::getenv("LQ_SPLIT_FALLBACK")) e.split_fallback.store(std::atoi(s));
         if (const char *s = std::getenv("LQ_MFMA_DENSE5")) e.mfma_dense5.store(std::atoi(s));
         if (const char *s = std::getenv("LQ_PRODUCT_FOLD")) e.product_fold.store(std::atoi(s) != 0);
+        if (const char *s = std::getenv("LQ_FIRST_BUDGET")) e.first_budget = std::atof(s);
         if (const char *s = std::getenv("LQ_PLAN_SEARCH")) e.plan_search.store(std::atoi(s) != 0);
         return p;
     }();
@@ -445,18 +446,19 @@ std::vector<cd> StateVector::run(const std::vector<LOp> &prog, const std::vector
                 pend_.swap(f);
                 return std::vector<cd>(static_cast<size_t>(nslots), 0.0);
             }
-            Planned folded = plan(rest, 1);
+            Planned folded = plan(rest, 1, true);
             if (n_ >= kPlanSearchQubits && tuning().plan_search.load()) {
                 // both forms let the first pass form the state without reading
                 // it; folding removes ops but can cost the greedy planner a pass
-                Planned whole = plan(prog, 1);
+                Planned whole = plan(prog, 1, true);
                 if (better_plan(whole.passes, folded.passes)) return execute(whole, states, nslots);
             }
             pend_.swap(f);
             return execute(folded, states, nslots);
         }
     }
-    return execute(plan(prog, static_cast<int>(states.size())), states, nslots);
+    const bool forms = prep_pending_ && states.size() == 1 && states[0] == d_;
+    return execute(plan(prog, static_cast<int>(states.size()), forms), states, nslots);
 }
 
 namespace {
@@ -564,7 +566,7 @@ std::vector<PassPlan> plan_search(const std::vector<LOp> &prog, const PlanConfig
     return passes;
 }
 
-StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
+StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns, bool forms) {
     LQ_ABORT_IF(ns < 1 || ns > kMaxStates, "too many states in one program");
     Planned out;
     out.ns = ns;
@@ -619,6 +621,7 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns) {
         // complex64: keep the unscaled amplitudes (at most 1) well inside float range
         scfg.max_scale_growth = dtype_ ? 1e300 : 1e20;
         scfg.lookahead = t.lookahead;
+        if (forms && ns == 1 && t.first_budget > 0) scfg.first_budget = t.first_budget;
         std::vector<LOp> split = prog;
         split_1q_rotations(split);
         passes = (t.budget <= 0 && t.plan_search.load() && n_ >= kPlanSearchQubits) ? plan_search(split, scfg)

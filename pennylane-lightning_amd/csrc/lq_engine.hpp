@@ -120,6 +120,9 @@ struct EngineTuning {
     // slower on the 30-qubit HEA (the greedy planner's passes came out
     // heavier: 124-141 ms against 119.5 ms per circuit), so they are off by
     // default (LQ_PRODUCT_FOLD=1, LQ_PLAN_SEARCH=1).
+    // > 0: budget of a specialised program's first pass when that pass forms
+    // a deferred state (LQ_FIRST_BUDGET)
+    double first_budget = -1.0;
     std::atomic<int> product_fold{0};
     std::atomic<int> plan_search{0};
 };
@@ -207,7 +210,8 @@ class StateVector {
         std::vector<const std::vector<uint64_t> *> geo; // its bit-position table (null or empty: none)
         int ns = 0;
     };
-    Planned plan(const std::vector<LOp> &prog, int ns);
+    // forms: the program starts from a deferred state its first pass forms
+    Planned plan(const std::vector<LOp> &prog, int ns, bool forms = false);
     std::vector<cd> execute(const Planned &pl, const std::vector<void *> &states, int nslots);
 
     // out = O in (observable applied, Hamiltonians summed); in/out device states

@@ -99,6 +99,44 @@ __device__ __forceinline__ void accp(double &r, double &i, const cplx b, const c
     i += g ? ti : 0.0;
 }
 __device__ __forceinline__ unsigned pc(unsigned long long x) { return (unsigned)__popcll(x); }
+// the amplitude of the lane whose index differs in the bits of m
+__device__ __forceinline__ cplx shx(const cplx a, const int m) {
+    return mk((real)__shfl_xor(a.x, m, 64), (real)__shfl_xor(a.y, m, 64));
+}
+// Exchange between the lanes 2^bit apart (bit 4 or 5): afterwards the lanes
+// with that lane bit clear hold in b their partner's a, the others hold in a
+// their partner's b (a register bit swapped with a lane bit). One
+// v_permlane16_swap / v_permlane32_swap per 32-bit word: it swaps the odd
+// 16-lane rows (upper 32 lanes) of its first operand with the even rows
+// (lower 32 lanes) of its second.
+#ifdef LQ_EMU
+__device__ __forceinline__ void pswap(cplx &a, cplx &b, const int bit) {
+    const bool hb = ((threadIdx.x >> bit) & 1u) != 0u;
+    const cplx x = shx(hb ? a : b, 1 << bit);
+    if (hb) a = x;
+    else b = x;
+}
+#else
+__device__ __forceinline__ void pswap(cplx &a, cplx &b, const int bit) {
+    unsigned wa[sizeof(cplx) / 4], wb[sizeof(cplx) / 4];
+    __builtin_memcpy(wa, &a, sizeof(cplx));
+    __builtin_memcpy(wb, &b, sizeof(cplx));
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(cplx) / 4); ++i) {
+        if (bit == 5) {
+            const auto r = __builtin_amdgcn_permlane32_swap(wa[i], wb[i], false, false);
+            wa[i] = r[0];
+            wb[i] = r[1];
+        } else {
+            const auto r = __builtin_amdgcn_permlane16_swap(wa[i], wb[i], false, false);
+            wa[i] = r[0];
+            wb[i] = r[1];
+        }
+    }
+    __builtin_memcpy(&a, wa, sizeof(cplx));
+    __builtin_memcpy(&b, wb, sizeof(cplx));
+}
+#endif
 // a zero the compiler cannot see through: coefficient offsets depend on it,
 // so coefficient loads stay next to their use in each tile instead of being
 // hoisted out of the tile loop (and kept in registers across it)
@@ -384,6 +422,112 @@ class Gen {
             if (a.r[i] != b.r[i]) return false;
         return true;
     }
+    bool same_regs(const Mapping &a, const Mapping &b) const {
+        for (int i = 0; i < rb_; ++i)
+            if (a.r[i] != b.r[i]) return false;
+        return true;
+    }
+    // the 16 lanes of a run hold the tile's four contiguous low bits in order
+    bool lanes_coalesce(const Mapping &m) const {
+        if (geom_.c < 4) return false;
+        for (int j = 0; j < 4; ++j)
+            if (m.h[j] != j) return false;
+        return true;
+    }
+
+    // A change of register bits made by swapping register bits with lane
+    // bits (thread bits 0..5) through wave shuffles: no LDS and no barrier.
+    // It applies when every bit entering the registers is a lane bit; the
+    // bit leaving register i takes the lane position of the bit entering, so
+    // the thread order of the result differs from the target's (only the
+    // register order must match: ops address registers by index).
+    struct LaneSwaps {
+        int n = 0;
+        int reg[kMaxRegBits], lane[kMaxRegBits];
+        Mapping result{};
+    };
+    bool plan_lane_swaps(const Mapping &to, LaneSwaps *ls) const {
+        const int mode = jit_config().lane_swaps;
+        if (mode == 0) return false;
+        int outs[kMaxRegBits], nout = 0, lanes[kMaxRegBits], nin = 0;
+        for (int i = 0; i < rb_; ++i) {
+            bool stays = false;
+            for (int k = 0; k < rb_; ++k) stays = stays || cur_.r[i] == to.r[k];
+            if (!stays) outs[nout++] = i;
+        }
+        for (int k = 0; k < rb_; ++k) {
+            bool held = false;
+            for (int i = 0; i < rb_; ++i) held = held || cur_.r[i] == to.r[k];
+            if (held) continue;
+            int j = -1;
+            for (int q = 0; q < 8; ++q)
+                if (cur_.h[q] == to.r[k]) j = q;
+            if (j < 0 || j >= 6) return false;
+            if (mode == 1 && j < 4) return false; // only the row swaps (lane bits 4, 5)
+            lanes[nin++] = j;
+        }
+        if (nin != nout) return false;
+        Mapping m = cur_;
+        for (int p = 0; p < nin; ++p) {
+            ls->reg[p] = outs[p];
+            ls->lane[p] = lanes[p];
+            std::swap(m.r[outs[p]], m.h[lanes[p]]);
+        }
+        ls->n = nin;
+        for (int k = 0; k < rb_; ++k) m.r[k] = to.r[k];
+        ls->result = m;
+        return true;
+    }
+    void emit_lane_swaps(const Mapping &to, const LaneSwaps &ls) {
+        std::ostringstream &o = o_;
+        Mapping m = cur_;
+        for (int p = 0; p < ls.n; ++p) {
+            const int i = ls.reg[p], j = ls.lane[p];
+            o << "  { // register bit " << i << " <-> lane bit " << j << "\n";
+            if (j < 4) o << "    const bool hb = ((tid >> " << j << ") & 1u) != 0u;\n";
+            for (int s = 0; s < ns_; ++s) {
+                if (!((p_.load_mask >> s) & 1u)) continue;
+                for (int e = 0; e < E_; ++e) {
+                    if ((e >> i) & 1) continue;
+                    const std::string a = "v" + std::to_string(s) + "[" + std::to_string(e) + "]";
+                    const std::string b = "v" + std::to_string(s) + "[" + std::to_string(e | (1 << i)) + "]";
+                    if (j >= 4) {
+                        o << "    pswap(" << a << ", " << b << ", " << j << ");\n";
+                        continue;
+                    }
+                    o << "    { const cplx x = shx(sel(hb, " << a << ", " << b << "), " << (1 << j) << ");\n";
+                    o << "      " << a << " = sel(hb, x, " << a << "); " << b << " = sel(hb, " << b << ", x); }\n";
+                }
+            }
+            o << "  }\n";
+            std::swap(m.r[i], m.h[j]);
+        }
+        // the target's register order: a renaming of the register arrays
+        int from[kMaxRegBits];
+        bool same = true;
+        for (int k = 0; k < rb_; ++k) {
+            from[k] = -1;
+            for (int i = 0; i < rb_; ++i)
+                if (m.r[i] == to.r[k]) from[k] = i;
+            LQ_ABORT_IF(from[k] < 0, "kernel generator: register bits lost in a lane swap");
+            same = same && from[k] == k;
+        }
+        if (!same) {
+            for (int s = 0; s < ns_; ++s) {
+                if (!((p_.load_mask >> s) & 1u)) continue;
+                o << "  {\n    cplx w[" << E_ << "];\n";
+                for (int e = 0; e < E_; ++e) {
+                    int em = 0;
+                    for (int k = 0; k < rb_; ++k)
+                        if ((e >> k) & 1) em |= 1 << from[k];
+                    o << "    w[" << e << "] = v" << s << "[" << em << "];\n";
+                }
+                for (int e = 0; e < E_; ++e) o << "    v" << s << "[" << e << "] = w[" << e << "];\n";
+                o << "  }\n";
+            }
+        }
+        cur_ = ls.result;
+    }
     // stages, store and dot partials of the tile `tile` held in v* (loaded in mapping lm)
     void emit_tile_body(const Mapping &m0, int nslots, const Mapping &lm) {
         std::ostringstream &o = o_;
@@ -401,21 +545,38 @@ class Gen {
             Mapping m{};
             for (int i = 0; i < rb_; ++i) m.r[i] = st.rbits[i];
             for (int j = 0; j < 8; ++j) m.h[j] = st.hbits[j];
-            if (st.entry == ENTRY_TRANSPOSE && !same_mapping(cur_, m)) transpose(m);
-            cur_ = m;
+            if (st.entry == ENTRY_TRANSPOSE && !same_regs(cur_, m)) {
+                LaneSwaps ls;
+                if (plan_lane_swaps(m, &ls)) {
+                    emit_lane_swaps(m, ls);
+                } else {
+                    transpose(m);
+                    cur_ = m;
+                }
+            }
             o << "  {\n";
-            emit_gl("gt", m);
+            emit_gl("gt", cur_);
             for (uint32_t oi = st.op_begin; oi < st.op_end; ++oi) emit_op(p_.ops[oi], static_cast<int>(oi));
             o << "  }\n";
         }
         if (p_.store_mask) {
             // store straight from the last stage's mapping when it coalesces
             const char *gs = "gl0";
+            LaneSwaps ls;
             if (p_.end_transpose && coalesces(cur_)) {
                 emit_gl("gls", cur_);
                 gs = "gls";
+            } else if (!same_regs(cur_, m0) && plan_lane_swaps(m0, &ls) && lanes_coalesce(ls.result)) {
+                // to the store's register bits by lane swaps; the lanes keep
+                // the low bits, so the stores coalesce in that mapping
+                emit_lane_swaps(m0, ls);
+                emit_gl("gls", cur_);
+                gs = "gls";
+            } else if (same_regs(cur_, m0) && !same_mapping(cur_, m0) && lanes_coalesce(cur_)) {
+                emit_gl("gls", cur_);
+                gs = "gls";
             } else {
-                if (p_.end_transpose) transpose(m0);
+                if (!same_mapping(cur_, m0)) transpose(m0);
                 cur_ = m0;
             }
             for (int s = 0; s < ns_; ++s) {
@@ -912,6 +1073,7 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_WAVES")) j.waves_per_eu = std::atoi(s);
         if (const char *s = std::getenv("LQ_JIT_DIRECT")) j.direct_io = std::atoi(s);
         if (const char *s = std::getenv("LQ_JIT_LIFT")) j.lift = std::atoi(s) != 0;
+        if (const char *s = std::getenv("LQ_JIT_LANES")) j.lane_swaps = std::atoi(s);
         return j;
     }();
     return c;

@@ -40,6 +40,11 @@ struct JitConfig {
                               // positions or qubit counts share a kernel (LQ_JIT_LIFT=1). Off:
                               // on the serving stream's 5315 passes (n 17..28, depth 4..16) it
                               // cuts distinct kernels only from 3104 to 2331
+    int lane_swaps = 0;       // register-bit changes whose incoming bits are lane bits, without
+                              // LDS: 1 only lane bits 4 and 5 (v_permlane16/32_swap), 2 any
+                              // lane bit (bits 4, 5 as 1; lower bits by ds_bpermute shuffles),
+                              // 0 none (LQ_JIT_LANES). Measured on the 30-qubit HEA: 1 no faster
+                              // than the LDS transposes, 2 slower; off by default
     int direct_io = 1;        // loads/stores in the first/last stage's mapping (no entry/exit
                               // transpose): 0 never, 1 where measured to pay, 2 whenever they coalesce
 };

@@ -1006,6 +1006,7 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
     // pass, where adding the first ready op's bits picks every other wire and
     // advances two.
     std::vector<int> touched;
+    double budget = cfg.budget; // this pass's
     auto run_with = [&](uint64_t T, std::vector<int> *frontier, bool *full) {
         std::priority_queue<int, std::vector<int>, std::greater<int>> q(ready.begin(), ready.end());
         double cost = 0.0;
@@ -1018,7 +1019,7 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
                 if (frontier) frontier->push_back(i);
                 continue;
             }
-            if (done > 0 && cost + ops[i].cost() > cfg.budget) {
+            if (done > 0 && cost + ops[i].cost() > budget) {
                 *full = true;
                 break;
             }
@@ -1069,6 +1070,7 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
     while (remaining > 0) {
         std::vector<int> sel;
         uint64_t T = 0;
+        budget = (passes.empty() && cfg.first_budget > 0) ? cfg.first_budget : cfg.budget;
         const uint64_t T0 = cfg.lookahead ? choose_bits() : low_mask;
         for (int attempt = 0; attempt < 2 && sel.empty(); ++attempt) {
             T = attempt == 0 ? T0 : 0ull;
@@ -1089,7 +1091,7 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
                             break;
                         }
                 if (pick < 0) break;
-                if (!sel.empty() && cost + ops[pick].cost() > cfg.budget) break;
+                if (!sel.empty() && cost + ops[pick].cost() > budget) break;
                 T |= nm[pick];
                 cost += ops[pick].cost();
                 dots += ops[pick].is_dot() ? 1 : 0;
@@ -1210,6 +1212,7 @@ std::string plan_structure_key(const std::vector<LOp> &ops, const PlanConfig &cf
                             cfg.scale_rotations ? 1 : 0, cfg.lookahead ? 1 : 0};
     put(hdr, sizeof(hdr));
     put(&b, sizeof(b));
+    put(&cfg.first_budget, sizeof(cfg.first_budget));
     put(&cfg.max_scale_growth, sizeof(cfg.max_scale_growth));
     const cd one(1.0), mone(-1.0);
     for (const LOp &o : ops) {

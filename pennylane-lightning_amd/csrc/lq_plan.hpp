@@ -116,6 +116,10 @@ struct PlanConfig {
     int kt_max = 12;     // max tile bits
     int low_bits = 4;    // contiguous low bits always in the tile
     double budget = 24;  // max op cost per pass
+    // > 0: the first pass's budget (a pass that forms a deferred state
+    // writes the state without reading it, so it has half the memory time to
+    // hide its arithmetic behind)
+    double first_budget = -1;
     int reg_rb = 0;      // > 0: emit register stages for tiles of 8 + reg_rb bits
     int max_dots = 1 << 30; // dot slots per pass
     // In one-state register passes without dots, store each real rotation

@@ -35,7 +35,7 @@ struct ServerConfig {
     int dtype = 1;
     int batched = 1;                 // 0: plain FIFO through the engine
     uint64_t hbm_budget = 250ull << 30;
-    int small_max_qubits = -1;       // -1: the packed kernel's limit
+    int small_max_qubits = -1;       // -1: the sizes whose states fit the packed kernel's LDS
     int max_batch = 4096;            // requests per packed launch
     int batch_streams = 2;           // packed launches in flight
     int large_workers = 4;

@@ -176,7 +176,8 @@ class Server:
                                    en.ctypes.data_as(_DP), gr.ctypes.data_as(_DP), lat.ctypes.data_as(_DP),
                                    path.ctypes.data_as(_IP), C.byref(wall)))
         grads = [gr[int(off[i]):int(off[i + 1])] for i in range(count)]
-        return {"energies": en, "grads": grads, "latency_ms": lat, "path": path, "wall_s": wall.value}
+        return {"energies": en, "grads": grads, "latency_ms": lat, "path": path, "wall_s": wall.value,
+                "arrivals_s": arr.copy()}
 
     def stats(self) -> dict:
         out = np.zeros(6, dtype=np.uint64)
@@ -186,7 +187,21 @@ class Server:
 
 
 def summarize(res: dict, count: int) -> dict:
+    """Latency percentiles over every request; ``req_per_s`` is all requests
+    over the whole replay (first arrival to last completion), and
+    ``steady_req_per_s`` the completions inside the arrival window (first to
+    last arrival) per second of that window -- the departure rate while
+    requests arrive, which equals the offered rate when the server keeps up
+    and is not diluted by the service time of the last few requests."""
     lat = np.asarray(res["latency_ms"])
-    return {"requests": count, "wall_s": res["wall_s"], "req_per_s": count / res["wall_s"] if res["wall_s"] else None,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "packed": int((np.asarray(res["path"]) == 0).sum())}
+    out = {"requests": count, "wall_s": res["wall_s"], "req_per_s": count / res["wall_s"] if res["wall_s"] else None,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "packed": int((np.asarray(res["path"]) == 0).sum())}
+    arr = res.get("arrivals_s")
+    if arr is not None and len(arr) > 1:
+        arr = np.asarray(arr, dtype=np.float64)
+        window = float(arr[-1] - arr[0])
+        done = arr + lat * 1e-3
+        if window > 0:
+            out["steady_req_per_s"] = float((done <= arr[-1]).sum()) / window
+    return out

@@ -90,7 +90,7 @@ static std::string transform(std::string s) {
     rep("typedef __attribute__((address_space(4))) const double2 kcoef;", "typedef const double2 kcoef;");
     rep("typedef __attribute__((address_space(4))) const unsigned long long kgeo;", "typedef const unsigned long long kgeo;");
     rep("typedef real nvec __attribute__((ext_vector_type(2)));", "struct nvec { real x, y; };");
-    return "#include \"emu_rt.hpp\"\n" + s;
+    return "#define LQ_EMU 1\n#include \"emu_rt.hpp\"\n" + s;
 }
 
 using Launch = void (*)(void *, void *, const double2 *, double2 *, unsigned long long, unsigned);

@@ -7,6 +7,10 @@ these sizes):
   |+>^n. Tolerance 1e-10.
 * 30 qubits: the complex64 run of the depth-20 HEA against the complex128 run:
   fidelity and expvals within the stated complex64 bounds (1e-5).
+* 33 qubits: the complex64 run (64 GiB) of the QFT + Haar brickwork +
+  3-controlled X circuit against the complex128 run (128 GiB, both resident):
+  fidelity, every <Z_i> and the marginal probabilities of wires 0-9 within
+  the stated complex64 bound of that configuration (1e-5).
 """
 import numpy as np
 import pytest
@@ -54,4 +58,21 @@ def test_hea30_complex64_against_complex128():
     ma, mb = L.MeasurementsC128(a), L.MeasurementsC64(b)
     assert abs(ma.expval("PauliX", [29]) - mb.expval("PauliX", [29])) < 1e-5
     assert abs(ma.expval("PauliZ", [0]) - mb.expval("PauliZ", [0])) < 1e-5
+    del ma, mb, a, b
+
+
+def test_qft33_complex64_against_complex128():
+    n = 33
+    ops = W.qft_brickwork_mcx(n, layers=8, mcx_per_layer=1, seed=5)
+    a = L.StateVectorC128(n)
+    W.apply_circuit(a, ops)
+    b = L.StateVectorC64(n)
+    W.apply_circuit(b, ops)
+    fid = abs(a.overlap(b)) ** 2 / b.overlap(b).real
+    assert fid > 1 - 1e-5
+    ma, mb = L.MeasurementsC128(a), L.MeasurementsC64(b)
+    za = np.asarray(ma.expval_named_batch(["PauliZ"] * n, list(range(n))))
+    zb = np.asarray(mb.expval_named_batch(["PauliZ"] * n, list(range(n))))
+    np.testing.assert_allclose(zb, za, atol=1e-5)
+    np.testing.assert_allclose(mb.probs(list(range(10))), ma.probs(list(range(10))), atol=1e-5)
     del ma, mb, a, b

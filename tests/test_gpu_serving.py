@@ -1,6 +1,6 @@
 """Request server: packed and engine paths against the oracle, each result the
-same alone and in a mix of concurrent requests, and the plain path against the
-batched one.
+same alone and in a mix of concurrent requests (up to 22 qubits), and the
+plain path against the batched one.
 
 Tolerances (complex128): energies and gradients 1e-10 against the oracle;
 alone vs in the mix 1e-12 (the same program runs either way; the engine path
@@ -42,7 +42,8 @@ def small_requests(count, seed, n_lo=3, n_hi=12, bits=128):
 def test_packed_against_oracle(bits, n_lo, n_hi):
     # up to 12 qubits (complex128) the states stay in LDS, above in global scratch
     reqs = small_requests(12 if n_hi <= 12 else 5, seed=3, n_lo=n_lo, n_hi=n_hi, bits=bits)
-    srv = S.Server(bits=bits, batched=True)
+    # (the server packs only LDS-resident sizes by default; allow the global-scratch form here)
+    srv = S.Server(bits=bits, batched=True, small_max_qubits=16)
     res = srv.replay(reqs, np.zeros(len(reqs)))
     st = srv.stats()
     srv.close()
@@ -58,19 +59,36 @@ def test_mixed_paths_alone_and_in_the_mix():
     reqs = small_requests(8, seed=5, n_lo=4, n_hi=12)
     reqs += small_requests(2, seed=6, n_lo=13, n_hi=16)
     reqs += small_requests(3, seed=7, n_lo=17, n_hi=17)  # engine path
-    srv = S.Server(bits=128, batched=True, large_workers=2, batch_streams=2)
+    reqs += small_requests(1, seed=8, n_lo=20, n_hi=20) + small_requests(1, seed=9, n_lo=22, n_hi=22)
+    srv = S.Server(bits=128, batched=True, large_workers=2, batch_streams=2, small_max_qubits=16)
     alone = [srv.run(r) for r in reqs]
     mix = srv.replay(reqs, np.linspace(0.0, 0.002, len(reqs)))
     st = srv.stats()
     srv.close()
-    assert st["engine"] == 6 and st["packed"] == 20
+    assert st["engine"] == 10 and st["packed"] == 20
     for i, r in enumerate(reqs):
         assert abs(mix["energies"][i] - alone[i][0]) < 1e-12
         np.testing.assert_allclose(mix["grads"][i], alone[i][1], atol=1e-12)
+        if r.n > 17:
+            continue  # alone == in the mix is the check at 20 and 22 qubits
         e, g = oracle_result(r)
         assert abs(alone[i][0] - e) < 1e-10
         np.testing.assert_allclose(alone[i][1], g, atol=1e-10)
     assert set(mix["path"][:10].tolist()) == {0} and set(mix["path"][10:].tolist()) == {1}
+
+
+def test_default_packs_lds_sizes_only():
+    """By default 13- to 16-qubit requests (complex128) take the engine path."""
+    reqs = small_requests(3, seed=4, n_lo=6, n_hi=12) + small_requests(2, seed=5, n_lo=13, n_hi=15)
+    srv = S.Server(bits=128, batched=True)
+    res = srv.replay(reqs, np.zeros(len(reqs)))
+    st = srv.stats()
+    srv.close()
+    assert st["packed"] == 3 and st["engine"] == 2
+    for i, r in enumerate(reqs):
+        e, g = oracle_result(r)
+        assert abs(res["energies"][i] - e) < 1e-10
+        np.testing.assert_allclose(res["grads"][i], g, atol=1e-10)
 
 
 def test_plain_matches_batched():

@@ -38,7 +38,8 @@ def emu_binary(tmp_path_factory):
     (0, 1, 0, 1, 14), (1, 1, 0, 1, 14), (2, 1, 0, 1, 14), (1, 0, 0, 1, 14), (2, 0, 0, 1, 14),
     (0, 1, 1, 1, 14), (2, 1, 1, 1, 14), (2, 0, 1, 1, 14), (0, 1, 0, 0, 14), (1, 0, 0, 0, 14), (2, 1, 0, 0, 14),
     (0, 1, 1, 2, 16), (1, 0, 1, 2, 16), (2, 1, 1, 2, 16), (0, 1, 1, 3, 14), (1, 0, 1, 3, 14),
-    (3, 0, 1, 1, 17), (3, 0, 1, 3, 17), (3, 1, 1, 3, 17), (2, 1, 1, 4, 14), (1, 0, 1, 4, 14), (0, 1, 1, 5, 14)])
+    (3, 0, 1, 1, 17), (3, 0, 1, 3, 17), (3, 1, 1, 3, 17), (2, 1, 1, 4, 14), (1, 0, 1, 4, 14), (0, 1, 1, 5, 14),
+    (0, 1, 1, 6, 14), (2, 1, 1, 6, 14), (1, 0, 1, 6, 14), (1, 0, 1, 7, 14), (3, 0, 1, 7, 17)])
 def test_generated_kernels_match_naive_application(emu_binary, case, dtype, lds32, look, n):
     exe, work = emu_binary
     # lds32: transposes through 32 KiB of LDS (emulated at exactly the size the kernel declares);
@@ -51,11 +52,16 @@ def test_generated_kernels_match_naive_application(emu_binary, case, dtype, lds3
     env = dict(os.environ, LQ_JIT_LDS32=str(lds32), EMU_LOOKAHEAD=str(min(look, 1)),
                LQ_JIT_DIRECT="2" if look == 2 else "1")
     # look 4: bit positions read from the table after the coefficients (LQ_JIT_LIFT); look 5: that and
-    # the deferred reset
-    if look >= 4:
+    # the deferred reset; look 6: register bits swapped with any lane bit by shuffles (LQ_JIT_LANES=2);
+    # look 7: only lane bits 4 and 5, by row swaps (LQ_JIT_LANES=1)
+    if look in (4, 5):
         env["EMU_LIFT"] = "1"
     if look in (3, 5):
         env["EMU_BASIS"] = "5"
+    if look == 6:
+        env["LQ_JIT_LANES"] = "2"
+    if look == 7:
+        env["LQ_JIT_LANES"] = "1"
     r = subprocess.run([exe, str(case), str(dtype), str(n), "3", EMU, work], capture_output=True, text=True,
                        timeout=900, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
