@@ -786,6 +786,9 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
             ra.store_mask = pp.store_mask;
             ra.end_transpose = pp.end_transpose;
             ra.ns = static_cast<uint32_t>(ns);
+            ra.wide = 0;
+            for (const TileOp &t : pp.ops) ra.wide |= t.kind == TOP_DENSE2 ? 1u : 0u;
+            for (const TileStage &st : pp.stages) ra.wide |= st.kind == STAGE_LDS ? 1u : 0u;
             ra.ntiles = a.ntiles;
             ra.g = a.g;
         }

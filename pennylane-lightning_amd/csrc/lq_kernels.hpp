@@ -52,6 +52,7 @@ struct TileRegArgs {
     uint32_t store_mask;
     uint32_t end_transpose;   // last stage's mapping differs from the HBM mapping
     uint32_t ns;
+    uint32_t wide;            // the pass has two-qubit dense ops or LDS stages
     uint64_t ntiles;
     TileGeom g;
 };

@@ -306,7 +306,7 @@ __device__ __forceinline__ void reg_diag(Regs<T, NS, RB> &R, const TileOp &op, c
     }
 }
 
-template <typename T, int NS, int RB>
+template <typename T, int NS, int RB, bool WIDE>
 __device__ __forceinline__ void reg_apply(Regs<T, NS, RB> &R, const TileOp &op, const double2 *coef, uint64_t gbt,
                                           const uint64_t (&rg)[kMaxRegBits], double &accr, double &acci) {
     const uint32_t kind = op.kind;
@@ -319,6 +319,7 @@ __device__ __forceinline__ void reg_apply(Regs<T, NS, RB> &R, const TileOp &op, 
         default: break;
         }
     } else if (kind == TOP_DENSE2) {
+        if constexpr (WIDE) {
 #define LQ_D2(a, b)                                                                                          \
     case (a)*4 + (b):                                                                                        \
         if constexpr ((a) < RB && (b) < RB) reg_dense2<T, NS, RB, a, b>(R, op, coef, gbt, rg, accr, acci);   \
@@ -329,6 +330,7 @@ __device__ __forceinline__ void reg_apply(Regs<T, NS, RB> &R, const TileOp &op, 
         default: break;
         }
 #undef LQ_D2
+        }
     } else {
         reg_diag<T, NS, RB>(R, op, coef, gbt, rg, accr, acci);
     }
@@ -337,7 +339,10 @@ __device__ __forceinline__ void reg_apply(Regs<T, NS, RB> &R, const TileOp &op, 
 // Every global pointer is its own __restrict__ parameter: the program (ops,
 // stages, coefficients) is then provably unclobbered by the state stores and
 // is read with scalar loads through the scalar cache.
-template <typename T, int NS, int RB>
+// WIDE: two-qubit dense ops and LDS stages are compiled in; passes without
+// them run the narrow form, whose smaller register footprint lets more waves
+// share a SIMD.
+template <typename T, int NS, int RB, bool WIDE>
 __global__ __launch_bounds__(kBlock) void tile_reg_kernel(TileRegArgs a, typename VecT<T>::type *__restrict__ sv0,
                                                          typename VecT<T>::type *__restrict__ sv1,
                                                          const TileOp *__restrict__ ops,
@@ -445,13 +450,13 @@ __global__ __launch_bounds__(kBlock) void tile_reg_kernel(TileRegArgs a, typenam
             for (uint32_t oi = st.op_begin; oi < st.op_end; ++oi) {
                 const TileOp &op = ops[oi];
                 double accr = 0.0, acci = 0.0;
-                reg_apply<T, NS, RB>(R, op, coef, gbt, rg, accr, acci);
+                reg_apply<T, NS, RB, WIDE>(R, op, coef, gbt, rg, accr, acci);
                 if (op.flags & TF_DOT) {
                     wave_sum(accr, acci);
                     if ((tid & 63u) == 0) dred[tid >> 6][op.slot] = make_double2(accr, acci);
                 }
             }
-        } else {
+        } else if constexpr (WIDE) {
             if (!in_lds) {
                 __syncthreads();
                 write_regs();
@@ -525,8 +530,14 @@ template <typename T> void launch_tile_reg(const TileRegArgs &a, size_t lds_byte
     V *p0 = static_cast<V *>(a.sv[0]);
     V *p1 = a.ns > 1 ? static_cast<V *>(a.sv[1]) : nullptr;
 #define LQ_LAUNCH(NS_, RB_)                                                                                    \
-    hipLaunchKernelGGL((tile_reg_kernel<T, NS_, RB_>), grid, dim3(kBlock), lds_bytes, s, a, p0, p1, a.ops,   \
-                       a.stages, a.coef, a.partials)
+    do {                                                                                                       \
+        if (a.wide)                                                                                            \
+            hipLaunchKernelGGL((tile_reg_kernel<T, NS_, RB_, true>), grid, dim3(kBlock), lds_bytes, s, a, p0,  \
+                               p1, a.ops, a.stages, a.coef, a.partials);                                       \
+        else                                                                                                   \
+            hipLaunchKernelGGL((tile_reg_kernel<T, NS_, RB_, false>), grid, dim3(kBlock), lds_bytes, s, a, p0, \
+                               p1, a.ops, a.stages, a.coef, a.partials);                                       \
+    } while (0)
     if (a.ns == 1) {
         switch (rb) {
         case 3: LQ_LAUNCH(1, 3); break;
