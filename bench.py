@@ -212,6 +212,47 @@ def run_adjoint(args):
     print(json.dumps(out))
 
 
+def run_adjoint_multi(args):
+    """Several observables in one adjoint Jacobian: the 24-qubit, 6-layer
+    strongly entangling circuit (432 parameters) with one named observable
+    and with three (<Z_0>, <X_11>, <Y_23>), each call = reset + forward + the
+    full Jacobian (one row per observable). Three observables share one
+    backward sweep once that program's kernels are compiled."""
+    from lightning_amd import lightning_qubit_ops as L
+    from lightning_amd import workloads as W
+
+    n, layers = 24, 6
+    ops = W.strongly_entangling(n, layers, seed=2)
+    sv = L.StateVectorC128(n)
+    ops_s = W.to_ops(ops, 128)
+    tp = list(range(sum(1 for o in ops if o.params)))
+    adj = L.algorithms.AdjointJacobianC128()
+    named = [("PauliZ", 0), ("PauliX", 11), ("PauliY", 23)]
+    res = {}
+    for m in (1, 3):
+        obs = [L.NamedObsC128(nm, [w]) for nm, w in named[:m]]
+
+        def step():
+            sv.resetStateVector()
+            W.apply_circuit(sv, ops)
+            return adj(sv, obs, ops_s, tp)
+
+        for _ in range(max(args.warmup, 1)):
+            step()
+        L.jit_wait()
+        step()
+        sv.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        sv.synchronize()
+        res[m] = (time.perf_counter() - t0) * 1e3 / args.steps
+    print(json.dumps({"metric": "adjoint Jacobian with 1 and 3 observables (24 qubits, 432 parameters)",
+                      "ms_one_observable": round(res[1], 3), "ms_three_observables": round(res[3], 3),
+                      "ratio": round(res[3] / res[1], 3), "n_gpus": 1, "dtype": "complex128", "data": "synthetic",
+                      "steps": args.steps}))
+
+
 def serving_sweep(count, qps_batched, qps_plain, bits=128, seed=0, server_kw=None):
     """Replay `count` requests at each arrival rate through the batched
     server, then through the plain path; returns the summary object."""
@@ -266,8 +307,8 @@ def run_serving(args):
     Hamiltonian; each asks for the energy and its full adjoint gradient
     (complex128). Poisson arrivals at each rate of a sweep, through the
     batched server and through the plain one-request-at-a-time path."""
-    qps_plain = [float(x) for x in (args.qps.split(",") if args.qps else ["40", "60", "80", "100", "120", "140"])]
-    qps_batched = [float(x) for x in (args.qps.split(",") if args.qps else ["80", "120", "160", "200", "240", "300"])]
+    qps_plain = [float(x) for x in (args.qps or args.sweep_plain or "40,60,80,100,120,140").split(",")]
+    qps_batched = [float(x) for x in (args.qps or args.sweep_batched or "80,120,160,200,240,300").split(",")]
     kw = {}
     if args.serve_small_max is not None:
         kw["small_max_qubits"] = args.serve_small_max
@@ -492,7 +533,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain streaming kernel (4 GiB read + 4 GiB written per launch)")
-    ap.add_argument("--workload", choices=["hea30", "adjoint24", "serving", "qft33", "precision", "mfma5"],
+    ap.add_argument("--workload", choices=["hea30", "adjoint24", "serving", "qft33", "precision", "mfma5",
+                                           "adjoint_multi"],
                     default="hea30",
                     help="hea30: the gate-application line (default); adjoint24: the 24-qubit adjoint gradient; "
                          "serving: energy + gradient requests, batched vs plain; qft33: 33-qubit QFT + Haar "
@@ -501,9 +543,11 @@ def main():
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--no-serving", action="store_true", help="default workload: skip the serving measurement")
     ap.add_argument("--serving-requests", type=int, default=1024)
-    ap.add_argument("--serving-qps-batched", default="150,225,300,375")
-    ap.add_argument("--serving-qps-plain", default="60,90,120,150")
+    ap.add_argument("--serving-qps-batched", default="300,450,550")
+    ap.add_argument("--serving-qps-plain", default="120,170,220")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
+    ap.add_argument("--sweep-batched", default="", help="serving workload: arrival rates of the batched server")
+    ap.add_argument("--sweep-plain", default="", help="serving workload: arrival rates of the plain path")
     ap.add_argument("--serve-small-max", type=int, default=None,
                     help="serving: largest qubit count run in the packed kernel (default: the server's)")
     ap.add_argument("--serve-workers", type=int, default=None,
@@ -536,6 +580,8 @@ def main():
         return run_precision(args)
     if args.workload == "mfma5":
         return run_mfma5(args)
+    if args.workload == "adjoint_multi":
+        return run_adjoint_multi(args)
     from lightning_amd import lightning_qubit_ops as L
     from lightning_amd import workloads as W
 

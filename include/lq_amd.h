@@ -66,6 +66,24 @@ int lq_sv_copy(lq_sv_t dst, lq_sv_t src);
 int lq_sv_synchronize(lq_sv_t sv);
 /* device address of the amplitudes (executes queued gates first) */
 int lq_sv_device_ptr(lq_sv_t sv, void **out);
+/* updateData / getState between the state and device memory of the caller
+ * (a torch tensor's storage, another library's buffer): len amplitudes of the
+ * state's precision, copied on the device. The caller's writes to src must be
+ * complete before the call; the copy is complete when the call returns.
+ * (Device-side form of lq/bindings/LQubitBindings.hpp:196-243.) */
+int lq_sv_update_data_device(lq_sv_t sv, const void *src, uint64_t len);
+int lq_sv_get_data_device(lq_sv_t sv, void *dst, uint64_t len);
+/* The state's amplitudes as a DLPack tensor without a copy: *out is a
+ * DLManagedTensor (dlpack.h v0.8 ABI; device kDLROCM, dtype complex64 or
+ * complex128, shape {2^n}, compact). Queued gates are executed and waited
+ * for first. The tensor shares the state's memory: gates applied later show
+ * in it. It keeps that memory alive after lq_sv_destroy until its deleter
+ * runs (the consumer calls it, or lq_dlpack_release for a tensor never
+ * consumed). The zero-copy counterpart of the reference's array exchange
+ * with numpy (core/bindings/Bindings.hpp:381-456, allocate_aligned_array;
+ * LQubitBindings.hpp:196-243, updateData / getState). */
+int lq_sv_dlpack(lq_sv_t sv, void **out);
+int lq_dlpack_release(void *managed);
 
 /* applyOperation(name, [controlled_wires, controlled_values,] wires, inverse, params)
  * (lq/StateVectorLQubit.hpp:378-423; binding core/bindings/Bindings.hpp:213-277).
@@ -282,8 +300,8 @@ int lq_set_split_fallback(int mode, int *previous);
 /* Matrix-core path for complex64 dense operators on exactly the five
  * lowest index bits (wires n-5 .. n-1 in any order) without controls:
  * with on != 0, applyMatrix of such an operator runs as one 64x64 real
- * GEMM per 32-amplitude group on the MFMA units (v_mfma_f32_32x32x2_f32)
- * instead of as a fused tile op; default off (LQ_MFMA_DENSE5). Replaces the
+ * GEMM per 32-amplitude group on the MFMA units (v_mfma_f32_16x16x4_f32)
+ * instead of as a fused tile op; default on (LQ_MFMA_DENSE5). Replaces the
  * dense k-qubit kernel for that case (lq/gates/cpu_kernels/
  * GateImplementationsLM.hpp:454-498, applyNQubitOp). */
 int lq_set_mfma_dense5(int on, int *previous);
@@ -293,7 +311,11 @@ int lq_set_mfma_dense5(int on, int *previous);
  * one-qubit gates of the next program fold into the product state its first
  * pass forms; default 0, LQ_PRODUCT_FOLD), "plan_search" (0/1: states of 26
  * qubits and more plan specialised programs under several pass budgets and
- * keep the plan with the fewest passes; default 0, LQ_PLAN_SEARCH). */
+ * keep the plan with the fewest passes; default 0, LQ_PLAN_SEARCH),
+ * "adjoint_group" (0: one observable per adjoint sweep; 1: up to three
+ * observables share a sweep once that sweep's specialised kernels are
+ * compiled and its estimated state traffic is below one sweep per
+ * observable; 2: share whenever compiled; default 1, LQ_ADJOINT_GROUP). */
 int lq_set_engine_option(const char *name, int value, int *previous);
 
 #ifdef __cplusplus

@@ -17,7 +17,9 @@
 using namespace lqamd;
 
 struct lq_sv_s {
-    std::unique_ptr<StateVector> sv;
+    // shared with the DLPack exports of the state, which keep its device
+    // memory alive after lq_sv_destroy until the consumer releases them
+    std::shared_ptr<StateVector> sv;
 };
 struct lq_obs_s {
     std::shared_ptr<Obs> obs;
@@ -46,6 +48,43 @@ StateVector &S(lq_sv_t h) {
     LQ_ABORT_IF(h == nullptr || !h->sv, "null state vector handle");
     return *h->sv;
 }
+
+// DLPack tensor exchange, ABI of dlpack.h v0.8 (DLManagedTensor; the form
+// torch.utils.dlpack.from_dlpack consumes): a 1-D compact tensor of 2^n
+// complex amplitudes on a ROCm device.
+struct DlDevice {
+    int32_t device_type;
+    int32_t device_id;
+};
+struct DlDataType {
+    uint8_t code;
+    uint8_t bits;
+    uint16_t lanes;
+};
+struct DlTensor {
+    void *data;
+    DlDevice device;
+    int32_t ndim;
+    DlDataType dtype;
+    int64_t *shape;
+    int64_t *strides;
+    uint64_t byte_offset;
+};
+struct DlManagedTensor {
+    DlTensor dl_tensor;
+    void *manager_ctx;
+    void (*deleter)(DlManagedTensor *);
+};
+constexpr int32_t kDlRocm = 10;
+constexpr uint8_t kDlComplex = 5;
+
+struct DlExport {
+    DlManagedTensor managed;
+    int64_t shape[1];
+    std::shared_ptr<StateVector> keep;
+};
+
+void dl_export_delete(DlManagedTensor *t) { delete static_cast<DlExport *>(t->manager_ctx); }
 } // namespace
 
 extern "C" {
@@ -61,7 +100,7 @@ int lq_device_count(int *out) {
 int lq_sv_create(uint32_t num_qubits, int dtype, lq_sv_t *out) {
     return guard([&] {
         auto h = std::make_unique<lq_sv_s>();
-        h->sv = std::make_unique<StateVector>(static_cast<int>(num_qubits), dtype);
+        h->sv = std::make_shared<StateVector>(static_cast<int>(num_qubits), dtype);
         *out = h.release();
     });
 }
@@ -111,6 +150,45 @@ int lq_sv_device_ptr(lq_sv_t sv, void **out) {
         // data() first: it may enqueue the write of a deferred reset
         *out = S(sv).data();
         S(sv).synchronize();
+    });
+}
+
+int lq_sv_update_data_device(lq_sv_t sv, const void *src, uint64_t len) {
+    return guard([&] { S(sv).update_data_device(src, len); });
+}
+
+int lq_sv_get_data_device(lq_sv_t sv, void *dst, uint64_t len) {
+    return guard([&] { S(sv).get_data_device(dst, len); });
+}
+
+int lq_sv_dlpack(lq_sv_t sv, void **out) {
+    return guard([&] {
+        LQ_ABORT_IF(out == nullptr, "null argument");
+        StateVector &s = S(sv);
+        void *p = s.data(); // executes a deferred reset's write first
+        s.synchronize();
+        hipPointerAttribute_t attr{};
+        hip_check(hipPointerGetAttributes(&attr, p), "hipPointerGetAttributes");
+        auto *e = new DlExport{};
+        e->keep = sv->sv;
+        e->shape[0] = static_cast<int64_t>(s.length());
+        e->managed.dl_tensor.data = p;
+        e->managed.dl_tensor.device = DlDevice{kDlRocm, attr.device};
+        e->managed.dl_tensor.ndim = 1;
+        e->managed.dl_tensor.dtype = DlDataType{kDlComplex, static_cast<uint8_t>(s.elem() * 8), 1};
+        e->managed.dl_tensor.shape = e->shape;
+        e->managed.dl_tensor.strides = nullptr;
+        e->managed.dl_tensor.byte_offset = 0;
+        e->managed.manager_ctx = e;
+        e->managed.deleter = dl_export_delete;
+        *out = &e->managed;
+    });
+}
+
+int lq_dlpack_release(void *managed) {
+    return guard([&] {
+        auto *t = static_cast<DlManagedTensor *>(managed);
+        if (t != nullptr && t->deleter != nullptr) t->deleter(t);
     });
 }
 
@@ -647,10 +725,13 @@ int lq_set_engine_option(const char *name, int value, int *previous) {
             opt = &t.product_fold;
         } else if (k == "plan_search") {
             opt = &t.plan_search;
+        } else if (k == "adjoint_group") {
+            LQ_ABORT_IF(value < 0 || value > 2, "adjoint_group must be 0, 1 or 2");
+            opt = &t.adjoint_group;
         } else {
             LQ_ABORT((std::string("unknown engine option ") + k).c_str());
         }
-        if (k != "split_fallback") value = value ? 1 : 0;
+        if (k != "split_fallback" && k != "adjoint_group") value = value ? 1 : 0;
         const int old = opt->exchange(value);
         if (previous) *previous = old;
     });

@@ -70,6 +70,10 @@ EngineTuning &tuning() {
         if (const char *s = std::getenv("LQ_PRODUCT_FOLD")) e.product_fold.store(std::atoi(s) != 0);
         if (const char *s = std::getenv("LQ_FIRST_BUDGET")) e.first_budget = std::atof(s);
         if (const char *s = std::getenv("LQ_PLAN_SEARCH")) e.plan_search.store(std::atoi(s) != 0);
+        if (const char *s = std::getenv("LQ_MULTI_BUDGET")) e.multi_budget = std::atof(s);
+        if (const char *s = std::getenv("LQ_PAIR_BUDGET")) e.pair_budget = std::atof(s);
+        if (const char *s = std::getenv("LQ_ADJOINT_GROUP")) e.adjoint_group.store(std::atoi(s));
+        if (const char *s = std::getenv("LQ_ADJOINT_DEBUG")) e.adjoint_debug = std::atoi(s) != 0;
         return p;
     }();
     return t;
@@ -392,6 +396,33 @@ void StateVector::get_data(void *host, uint64_t len) {
     synchronize();
 }
 
+// device-to-device copies between the state and caller memory (a torch
+// tensor's storage, another library's buffer): the streaming copy kernel
+// when both addresses are 16-byte aligned, the runtime's copy otherwise
+static void device_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
+    if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) == 0)
+        launch_stream_copy(src, dst, bytes, s);
+    else
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+}
+
+void StateVector::update_data_device(const void *src, uint64_t len) {
+    LQ_ABORT_IF(len != length(), "The size of provided data must be equal to the statevector size.");
+    LQ_ABORT_IF(src == nullptr, "null device address");
+    queue_.clear();
+    prep_pending_ = false; // every amplitude is overwritten
+    device_copy(src, d_, bytes(), stream_);
+    synchronize();
+}
+
+void StateVector::get_data_device(void *dst, uint64_t len) {
+    LQ_ABORT_IF(len != length(), "The size of provided data must be equal to the statevector size.");
+    LQ_ABORT_IF(dst == nullptr, "null device address");
+    flush();
+    device_copy(data(), dst, bytes(), stream_);
+    synchronize();
+}
+
 void StateVector::copy_from(StateVector &src) {
     LQ_ABORT_IF(src.n_ != n_ || src.dtype_ != dtype_, "state vectors differ in size or precision");
     if (&src == this) {
@@ -592,7 +623,11 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns, boo
     // when its kernels are ready; meanwhile their compiles are queued and the
     // unsplit plan runs in the generic kernel.
     const JitConfig &jcfg = jit_config();
-    const bool use_jit = cfg.reg_rb > 0 && jcfg.enabled && n_ >= jcfg.min_qubits;
+    // three or four states: register passes only as specialised kernels
+    // (tiles of 11 bits; the generic register kernel holds two states), the
+    // generic fallback being LDS passes
+    const bool multi = ns > 2 && t.reg && n_ >= 11 && jcfg.lds32;
+    const bool use_jit = (cfg.reg_rb > 0 || multi) && jcfg.enabled && n_ >= jcfg.min_qubits;
     std::vector<PassPlan> passes;
     std::vector<hipFunction_t> fns;
     std::vector<const std::vector<uint64_t> *> geo;
@@ -622,13 +657,27 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns, boo
         scfg.max_scale_growth = dtype_ ? 1e300 : 1e20;
         scfg.lookahead = t.lookahead;
         if (forms && ns == 1 && t.first_budget > 0) scfg.first_budget = t.first_budget;
+        // two states (one observable's adjoint sweep, a VJP): budget 78,
+        // measured on the 24-qubit / 432-parameter gradient (30 passes and
+        // 18.0 ms against 33 and 18.8 ms with 52; 104 the same as 78); three
+        // or four (observables sharing a sweep): 26 (ns + 1), since the
+        // inverse gates act on every state and the dots per op grow with the
+        // observables (three observables at 24 qubits: 130 gives 20 passes
+        // and 27.1 ms, 104 24 passes and 33.6 ms, 78 32 passes and 40.1 ms)
+        if (t.budget <= 0 && ns == 2) scfg.budget = t.pair_budget > 0 ? t.pair_budget : 78.0;
+        if (t.budget <= 0 && ns > 2) scfg.budget = t.multi_budget > 0 ? t.multi_budget : 26.0 * (ns + 1);
+        if (multi) {
+            scfg.kt_max = 11;
+            scfg.reg_rb = 3;
+            scfg.max_dots = std::min(kMaxJitPassDots, kMaxPassDots * (ns - 1));
+        }
         std::vector<LOp> split = prog;
         split_1q_rotations(split);
         passes = (t.budget <= 0 && t.plan_search.load() && n_ >= kPlanSearchQubits) ? plan_search(split, scfg)
                                                                               : plan_cached(split, scfg);
         const size_t missing = lookup_all(fns);
         const int fallback = t.split_fallback.load();
-        if (missing && fallback != 2) {
+        if (missing && (fallback != 2 || multi)) {
             // The generic kernel interprets the op list, so its pass time grows
             // with the ops: measured on the 30-qubit HEA passes, about 0.28 of
             // a specialised pass per op of the split plan (phase tables, real
@@ -639,7 +688,7 @@ StateVector::Planned StateVector::plan(const std::vector<LOp> &prog, int ns, boo
             for (size_t p = 0; p < passes.size(); ++p)
                 split_cost += (fns[p] || !passes[p].reg) ? 1.0 : 0.28 * static_cast<double>(passes[p].ops.size());
             for (const PassPlan &pp : plain) plain_cost += std::max(1.0, 0.16 * static_cast<double>(pp.ops.size()));
-            if (fallback == 0 || plain_cost < split_cost) {
+            if (fallback == 0 || multi || plain_cost < split_cost) {
                 passes.swap(plain);
                 fns.assign(passes.size(), nullptr);
                 geo.assign(passes.size(), nullptr);
@@ -773,8 +822,10 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         const size_t lds = static_cast<size_t>(ns) * (elem() << a.g.kt);
         TileRegArgs ra{};
         if (pp.reg) {
-            LQ_ABORT_IF(a.g.kt < 11 || a.g.kt > 8 + kMaxRegBits || ns > 2, "register pass geometry out of range");
-            LQ_ABORT_IF(pp.slot_ids.size() > static_cast<size_t>(kMaxPassDots), "too many dots in one pass");
+            LQ_ABORT_IF(a.g.kt < 11 || a.g.kt > 8 + kMaxRegBits || (ns > 2 && !fns[p]),
+                        "register pass geometry out of range");
+            LQ_ABORT_IF(pp.slot_ids.size() > static_cast<size_t>(fns[p] && ns > 2 ? kMaxJitPassDots : kMaxPassDots),
+                        "too many dots in one pass");
             for (int s = 0; s < ns; ++s) ra.sv[s] = states[s];
             ra.ops = a.ops;
             ra.stages = reinterpret_cast<const TileStage *>(dp + stages_at + st_off[p]);
@@ -808,8 +859,8 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
             hip_check(hipEventRecord(pe.a, stream_), "hipEventRecord");
         }
         if (fns[p])
-            jit_launch(fns[p], dtype_, ra.sv[0], ra.sv[1], ra.coef, ra.partials, ra.ntiles, jit_lds_bytes(a.g.kt, dtype_, ns),
-                       stream_);
+            jit_launch(fns[p], dtype_, ra.sv[0], ra.sv[1], ra.sv[2], ra.sv[3], ra.coef, ra.partials, ra.ntiles,
+                       jit_lds_bytes(a.g.kt, dtype_, ns), stream_);
         else if (pp.reg) dispatch(dtype_, [&](auto tt) { launch_tile_reg<decltype(tt)>(ra, lds, stream_); });
         else dispatch(dtype_, [&](auto tt) { launch_tile_pass<decltype(tt)>(a, lds, stream_); });
         ++(fns[p] ? jit_passes_ : generic_passes_);
@@ -1677,23 +1728,55 @@ void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std
     void *lam0 = sv.workspace(0);
     hip_check(hipMemcpyAsync(lam0, sv.data(), sv.bytes(), hipMemcpyDeviceToDevice, sv.stream()), "memcpy");
     if (apply_operations) apply_forward(sv, ops, lam0);
-    // one H|lambda> per sweep keeps two tiles in LDS; more observables re-run
-    // the sweep from a saved lambda. Every sweep runs the same program: it is
-    // built and planned once, on the host while the forward pass runs.
-    const size_t per_sweep = 1;
-    std::vector<LOp> prog;
-    int exp_slots = 0;
-    if (expvals) {
-        for (size_t j = 0; j < per_sweep; ++j) prog.push_back(identity_dot(0, 1 + static_cast<uint32_t>(j), exp_slots++));
+    // Observables share a sweep in groups of up to three (lambda and their
+    // H|lambda> in one program of up to four states, so the inverse gates
+    // on lambda run once per group) when that program's specialised kernels
+    // are ready; otherwise, and for the observables left over, one H|lambda>
+    // per sweep re-runs it from a saved lambda. Each program is built and
+    // planned on the host while the forward pass runs.
+    struct Sweep {
+        size_t m = 0;
+        int exp_slots = 0, nslots = 0;
+        std::vector<double> scales;
+        StateVector::Planned planned;
+    };
+    auto make_sweep = [&](size_t m) {
+        Sweep w;
+        w.m = m;
+        std::vector<LOp> prog;
+        if (expvals)
+            for (size_t j = 0; j < m; ++j) prog.push_back(identity_dot(0, 1 + static_cast<uint32_t>(j), w.exp_slots++));
+        w.scales = backward_sweep(sv.n(), ops, tp, m, w.exp_slots, prog);
+        w.nslots = w.exp_slots + static_cast<int>(ntp * m);
+        w.planned = sv.plan(prog, 1 + static_cast<int>(m));
+        return w;
+    };
+    Sweep single = make_sweep(1);
+    Sweep group;
+    const int grouping = tuning().adjoint_group.load();
+    if (nobs > 1 && grouping > 0) {
+        group = make_sweep(std::min<size_t>(3, nobs));
+        // state traffic of the sweeps (passes x states each pass streams):
+        // a group against one sweep per observable
+        const double g_cost = static_cast<double>(group.planned.passes.size()) * static_cast<double>(1 + group.m);
+        const double s_cost = static_cast<double>(single.planned.passes.size()) * 2.0 * static_cast<double>(group.m);
+        const bool ready = group.planned.all_specialised();
+        const bool take = ready && (grouping == 2 || g_cost < s_cost);
+        if (tuning().adjoint_debug)
+            std::fprintf(stderr, "[lq adjoint] observables %zu: group of %zu in %zu passes (specialised %d), single %zu "
+                                 "passes: %s\n",
+                         nobs, group.m, group.planned.passes.size(), ready ? 1 : 0, single.planned.passes.size(),
+                         take ? "groups" : "one per sweep");
+        if (!take) group = Sweep{};
     }
-    const std::vector<double> scales = backward_sweep(sv.n(), ops, tp, per_sweep, exp_slots, prog);
-    const int nslots = exp_slots + static_cast<int>(ntp * per_sweep);
-    const StateVector::Planned planned = sv.plan(prog, 1 + static_cast<int>(per_sweep));
     std::vector<double> out(nobs * ntp, 0.0);
-    for (size_t o0 = 0; o0 < nobs; o0 += per_sweep) {
-        const size_t m = std::min(per_sweep, nobs - o0);
+    for (size_t o0 = 0; o0 < nobs;) {
+        const Sweep &w = (group.m > 0 && nobs - o0 >= group.m) ? group : single;
+        const size_t m = w.m;
+        // the sweep runs its inverse gates on lambda in place: groups with
+        // more to follow work on a copy, the last one on lambda itself
         void *lam = lam0;
-        if (nobs > per_sweep) {
+        if (o0 + m < nobs) {
             lam = sv.workspace(1);
             hip_check(hipMemcpyAsync(lam, lam0, sv.bytes(), hipMemcpyDeviceToDevice, sv.stream()), "memcpy");
         }
@@ -1703,12 +1786,13 @@ void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std
             sv.apply_obs_to(*obs[o0 + j], lam, h);
             states.push_back(h);
         }
-        const auto v = sv.execute(planned, states, nslots);
+        const auto v = sv.execute(w.planned, states, w.nslots);
         for (size_t j = 0; j < m; ++j) {
             if (expvals) expvals[o0 + j] = v[j].real();
             for (size_t t = 0; t < ntp; ++t)
-                out[(o0 + j) * ntp + t] = -2.0 * scales[t] * v[exp_slots + t * m + j].imag();
+                out[(o0 + j) * ntp + t] = -2.0 * w.scales[t] * v[w.exp_slots + t * m + j].imag();
         }
+        o0 += m;
     }
     std::copy(out.begin(), out.end(), jac);
 }

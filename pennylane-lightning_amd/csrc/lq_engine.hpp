@@ -125,6 +125,16 @@ struct EngineTuning {
     double first_budget = -1.0;
     std::atomic<int> product_fold{0};
     std::atomic<int> plan_search{0};
+    // pass budget of specialised programs on three or four states (the
+    // adjoint sweep of several observables; LQ_MULTI_BUDGET) and on two
+    // (LQ_PAIR_BUDGET); <= 0: 26 (ns + 1) and 78
+    double multi_budget = -1.0;
+    double pair_budget = -1.0;
+    // observables per adjoint sweep: 0 one; 1 groups of up to three when
+    // their specialised kernels are ready and the estimated state traffic
+    // is lower than one sweep per observable; 2 groups whenever ready
+    std::atomic<int> adjoint_group{1};
+    bool adjoint_debug = false; // LQ_ADJOINT_DEBUG: report each choice on stderr
 };
 EngineTuning &tuning();
 
@@ -154,6 +164,9 @@ class StateVector {
     void set_state_vector(const void *host, const uint32_t *wires, int nw);
     void update_data(const void *host, uint64_t len);
     void get_data(void *host, uint64_t len);
+    // the same from / into device memory of the caller (len amplitudes)
+    void update_data_device(const void *src, uint64_t len);
+    void get_data_device(void *dst, uint64_t len);
     void copy_from(StateVector &src);
 
     // gates (lq/StateVectorLQubit.hpp:378-710)
@@ -209,6 +222,11 @@ class StateVector {
         std::vector<hipFunction_t> fns; // specialised kernel per pass, or null
         std::vector<const std::vector<uint64_t> *> geo; // its bit-position table (null or empty: none)
         int ns = 0;
+        bool all_specialised() const {
+            for (hipFunction_t f : fns)
+                if (!f) return false;
+            return !fns.empty();
+        }
     };
     // forms: the program starts from a deferred state its first pass forms
     Planned plan(const std::vector<LOp> &prog, int ns, bool forms = false);

@@ -157,7 +157,7 @@ __device__ __forceinline__ void stnt(cplx *p, const cplx v) {
     __builtin_nontemporal_store(t, reinterpret_cast<nvec *>(p));
 }
 __device__ __forceinline__ unsigned swz(unsigned l) { return LQ_SWZ; }
-__device__ __forceinline__ void dflush(double r, double i, double2 (*dred)[32], unsigned slot) {
+template <int D> __device__ __forceinline__ void dflush(double r, double i, double2 (*dred)[D], unsigned slot) {
     for (int off = 32; off > 0; off >>= 1) {
         r += __shfl_xor(r, off, 64);
         i += __shfl_xor(i, off, 64);
@@ -165,6 +165,9 @@ __device__ __forceinline__ void dflush(double r, double i, double2 (*dred)[32], 
     if ((threadIdx.x & 63u) == 0) dred[threadIdx.x >> 6][slot] = make_double2(r, i);
 }
 )";
+
+// row length of a pass's per-wave dot buffer: the slots rounded up to 32
+int dred_len(int nslots) { return std::max(32, (nslots + 31) / 32 * 32); }
 
 std::string hex64(uint64_t v) {
     std::ostringstream s;
@@ -233,10 +236,11 @@ class Gen {
         o << "extern \"C\" __global__ __launch_bounds__(256) ";
         if (const int w = jit_waves_per_eu(dtype_, ns_); w > 0) o << "__attribute__((amdgpu_waves_per_eu(" << w << "))) ";
         o << "void lqpass(cplx *__restrict__ sv0, cplx *__restrict__ sv1,"
-             " const double2 *__restrict__ coef, double2 *__restrict__ partials, unsigned long long ntiles) {\n";
+             " const double2 *__restrict__ coef, double2 *__restrict__ partials, unsigned long long ntiles,"
+             " cplx *__restrict__ sv2, cplx *__restrict__ sv3) {\n";
         o << "  extern __shared__ cplx lds[];\n";
         const int nslots = static_cast<int>(p_.slot_ids.size());
-        if (nslots) o << "  __shared__ double2 dred[4][32];\n";
+        if (nslots) o << "  __shared__ double2 dred[4][" << dred_len(nslots) << "];\n";
         o << "  const unsigned tid = threadIdx.x;\n";
         Mapping m0{};
         for (int i = 0; i < rb_; ++i) m0.r[i] = 8 + i;
@@ -535,7 +539,9 @@ class Gen {
         o << "  const unsigned long long base = " << base_expr("tile") << ";\n";
         o << "  const unsigned cz = opaque0();\n";
         if (nslots) {
-            o << "  for (unsigned i = tid; i < 128u; i += 256u) dred[i >> 5][i & 31u] = make_double2(0.0, 0.0);\n";
+            const int d = dred_len(nslots);
+            o << "  for (unsigned i = tid; i < " << 4 * d << "u; i += 256u) dred[i / " << d << "u][i % " << d
+              << "u] = make_double2(0.0, 0.0);\n";
             o << "  __syncthreads();\n";
         }
         cur_ = lm;
@@ -1028,7 +1034,12 @@ class Gen {
 std::string jit_pass_source(const PassPlan &p, int n, int dtype, int ns, std::vector<uint64_t> *geo) {
     if (!p.reg || p.stages.empty()) return "";
     const int kt = static_cast<int>(p.tbits.size());
-    if (kt < 11 || kt > 13 || ns > 2 || p.slot_ids.size() > 32) return "";
+    // three and four states (the adjoint's lambda and several H lambda) in
+    // tiles of 11 bits, eight amplitudes per state per thread, transposes
+    // through the 32 KiB LDS buffer one state at a time
+    if (kt < 11 || kt > 13 || ns > 4) return "";
+    if (p.slot_ids.size() > static_cast<size_t>(ns > 2 ? kMaxJitPassDots : kMaxPassDots)) return "";
+    if (ns > 2 && (kt != 11 || !jit_config().lds32)) return "";
     for (const auto &st : p.stages)
         if (st.kind != STAGE_REG) return "";
     for (const auto &t : p.ops) {
@@ -1314,12 +1325,12 @@ void jit_wait() { Service::get().wait(); }
 void jit_shutdown() { Service::get().stop(); }
 void jit_stats(int *ready, int *failed, int *pending) { Service::get().stats(ready, failed, pending); }
 
-void jit_launch(hipFunction_t f, int dtype, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
-                size_t lds_bytes, hipStream_t s) {
-    void *a0 = sv0, *a1 = sv1, *a3 = partials;
+void jit_launch(hipFunction_t f, int dtype, void *sv0, void *sv1, void *sv2, void *sv3, const void *coef,
+                void *partials, uint64_t ntiles, size_t lds_bytes, hipStream_t s) {
+    void *a0 = sv0, *a1 = sv1, *a3 = partials, *a5 = sv2, *a6 = sv3;
     const void *a2 = coef;
     unsigned long long nt = ntiles;
-    void *args[] = {&a0, &a1, &a2, &a3, &nt};
+    void *args[] = {&a0, &a1, &a2, &a3, &nt, &a5, &a6};
     const uint32_t gx = static_cast<uint32_t>(ntiles < 65536 ? ntiles : 65536);
     const uint32_t gy = static_cast<uint32_t>((ntiles + gx - 1) / gx);
     const hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, kBlock, 1, 1, static_cast<unsigned>(lds_bytes), s, args, nullptr);

@@ -72,7 +72,7 @@ void jit_shutdown();
 void jit_stats(int *ready, int *failed, int *pending);
 
 // Launch a specialised pass (arguments as the generic register kernel).
-void jit_launch(hipFunction_t f, int dtype, void *sv0, void *sv1, const void *coef, void *partials, uint64_t ntiles,
-                size_t lds_bytes, hipStream_t s);
+void jit_launch(hipFunction_t f, int dtype, void *sv0, void *sv1, void *sv2, void *sv3, const void *coef,
+                void *partials, uint64_t ntiles, size_t lds_bytes, hipStream_t s);
 
 } // namespace lqamd

@@ -41,7 +41,7 @@ template <typename T> void launch_tile_pass(const TilePassArgs &a, size_t lds_by
 // Register-resident tile pass (lq_kernels_reg.hip): kt = 8 + RB, RB in {3, 4},
 // one or two states.
 struct TileRegArgs {
-    void *sv[2];
+    void *sv[kMaxStates]; // the generic kernel takes two; specialised passes up to four
     const TileOp *ops;
     const TileStage *stages;
     const double2 *coef;

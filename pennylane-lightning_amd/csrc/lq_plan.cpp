@@ -1121,7 +1121,7 @@ std::vector<PassPlan> plan_passes(const std::vector<LOp> &ops, const PlanConfig 
             return r;
         };
         const int kt = static_cast<int>(p.tbits.size());
-        p.reg = cfg.reg_rb > 0 && kt == 8 + cfg.reg_rb && cfg.ns <= 2;
+        p.reg = cfg.reg_rb > 0 && kt == 8 + cfg.reg_rb && cfg.ns <= kMaxStates;
         std::vector<int> order = sel;
         std::vector<StagePlan> stages;
         if (p.reg) {

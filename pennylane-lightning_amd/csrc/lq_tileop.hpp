@@ -45,6 +45,10 @@ enum TileStageKind : uint32_t { STAGE_REG = 1, STAGE_LDS = 2 };
 enum TileStageEntry : uint32_t { ENTRY_NONE = 0, ENTRY_TRANSPOSE = 1, ENTRY_FROM_LDS = 2 };
 constexpr int kMaxRegBits = 4;    // 16 amplitudes per state per thread
 constexpr int kMaxPassDots = 32;  // dot slots per register-kernel pass
+// dot slots per specialised pass of a program on three or four states (the
+// adjoint sweep of several observables: each trainable op dots G lambda with
+// every observable's state, so such passes fill their slots 2-3x faster)
+constexpr int kMaxJitPassDots = 96;
 
 struct TileStage {
     uint32_t kind;

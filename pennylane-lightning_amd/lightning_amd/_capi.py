@@ -18,6 +18,16 @@ if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built: run `make -C pennylane-lightning_amd` "
                       "(or __graft_entry__.build()) first")
 
+# torch (when installed) is imported first: the engine's HIP runtime
+# libraries have the same sonames as the ones torch bundles, so the loader
+# binds the engine to torch's copies and the process has one HIP runtime.
+# Device memory then passes between the two without copies (DLPack), and
+# either may initialise the GPU first.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - the engine works without torch
+    pass
+
 lib = C.CDLL(LIB_PATH)
 
 _P = C.c_void_p
@@ -43,6 +53,10 @@ _SIGS = {
     "lq_sv_copy": (C.c_int, [_P, _P]),
     "lq_sv_synchronize": (C.c_int, [_P]),
     "lq_sv_device_ptr": (C.c_int, [_P, C.POINTER(_P)]),
+    "lq_sv_update_data_device": (C.c_int, [_P, _P, C.c_uint64]),
+    "lq_sv_get_data_device": (C.c_int, [_P, _P, C.c_uint64]),
+    "lq_sv_dlpack": (C.c_int, [_P, C.POINTER(_P)]),
+    "lq_dlpack_release": (C.c_int, [_P]),
     "lq_sv_apply_op": (C.c_int, [_P, C.c_char_p, _U32P, C.c_uint32, _U32P, _U8P, C.c_uint32, C.c_int, _DP,
                                  C.c_uint32]),
     "lq_sv_apply_operations": (C.c_int, [_P, C.c_uint32, C.c_char_p, _U32P, _U32P, _U8P, _U32P, _DP]),

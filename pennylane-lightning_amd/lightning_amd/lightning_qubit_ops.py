@@ -102,6 +102,70 @@ def _is_bool(x) -> bool:
     return isinstance(x, (bool, np.bool_))
 
 
+class _DLDevice(C.Structure):
+    _fields_ = [("device_type", C.c_int32), ("device_id", C.c_int32)]
+
+
+class _DLDataType(C.Structure):
+    _fields_ = [("code", C.c_uint8), ("bits", C.c_uint8), ("lanes", C.c_uint16)]
+
+
+class _DLTensor(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("device", _DLDevice), ("ndim", C.c_int32), ("dtype", _DLDataType),
+                ("shape", C.POINTER(C.c_int64)), ("strides", C.POINTER(C.c_int64)), ("byte_offset", C.c_uint64)]
+
+
+class _DLManagedTensor(C.Structure):
+    _fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", C.c_void_p), ("deleter", C.c_void_p)]
+
+
+_DLTENSOR = b"dltensor"
+_capsule_new = C.PYFUNCTYPE(C.py_object, C.c_void_p, C.c_char_p, C.c_void_p)(("PyCapsule_New", C.pythonapi))
+_capsule_is_valid = C.PYFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p)(("PyCapsule_IsValid", C.pythonapi))
+_capsule_pointer = C.PYFUNCTYPE(C.c_void_p, C.c_void_p, C.c_char_p)(("PyCapsule_GetPointer", C.pythonapi))
+
+
+@C.CFUNCTYPE(None, C.c_void_p)
+def _capsule_destructor(capsule):
+    # a capsule still named "dltensor" was never consumed: release the export
+    # (a consumer renames it and calls the tensor's deleter itself)
+    if _capsule_is_valid(capsule, _DLTENSOR):
+        lib.lq_dlpack_release(C.c_void_p(_capsule_pointer(capsule, _DLTENSOR)))
+
+
+def _dlpack_capsule(managed: int):
+    return _capsule_new(managed, _DLTENSOR, C.cast(_capsule_destructor, C.c_void_p))
+
+
+def _is_device_tensor(x) -> bool:
+    dev = getattr(x, "__dlpack_device__", None)
+    return dev is not None and hasattr(x, "data_ptr") and dev()[0] in (2, 10) and not isinstance(x, _StateVectorBase)
+
+
+def _torch_dtype_ok(t, cdtype) -> bool:
+    import torch
+    return t.dtype == (torch.complex64 if cdtype == np.complex64 else torch.complex128)
+
+
+def _device_source(t, cdtype, size):
+    """A torch GPU tensor as the source of a device copy: 1-D, contiguous, the
+    state's precision and size; its producer stream is waited for."""
+    import torch
+    if t.dim() != 1:
+        raise ValueError("Array must be 1-dimensional")
+    if not _torch_dtype_ok(t, cdtype) or not t.is_contiguous() or t.numel() != size:
+        raise LightningException("The size of provided data must be equal to the statevector size.")
+    torch.cuda.current_stream(t.device).synchronize()
+    return t
+
+
+def _check_device_target(t, cdtype, size):
+    import torch
+    if t.dim() != 1 or not _torch_dtype_ok(t, cdtype) or not t.is_contiguous() or t.numel() != size:
+        raise ValueError("output must be a contiguous 1-D tensor of the state's size and precision")
+    torch.cuda.current_stream(t.device).synchronize()
+
+
 class _StateVectorBase:
     _bits = 128
 
@@ -147,6 +211,10 @@ class _StateVectorBase:
         check(lib.lq_sv_set_state_vector(self._h, s.ctypes.data_as(C.c_void_p), wp, nw))
 
     def updateData(self, data):
+        if _is_device_tensor(data):
+            t = _device_source(data, self._cdtype, 1 << self._n)
+            check(lib.lq_sv_update_data_device(self._h, C.c_void_p(t.data_ptr()), C.c_uint64(t.numel())))
+            return
         a = np.asarray(data)
         if a.ndim != 1:
             raise ValueError("Array must be 1-dimensional")
@@ -154,6 +222,10 @@ class _StateVectorBase:
         check(lib.lq_sv_update_data(self._h, a.ctypes.data_as(C.c_void_p), C.c_uint64(a.size)))
 
     def getState(self, state):
+        if _is_device_tensor(state):
+            _check_device_target(state, self._cdtype, 1 << self._n)
+            check(lib.lq_sv_get_data_device(self._h, C.c_void_p(state.data_ptr()), C.c_uint64(1 << self._n)))
+            return
         if state.shape[0] < (1 << self._n):
             raise ValueError("Output array is too small")
         if state.dtype != self._cdtype or not state.flags.c_contiguous:
@@ -170,6 +242,32 @@ class _StateVectorBase:
 
     def synchronize(self):
         check(lib.lq_sv_synchronize(self._h))
+
+    # -- zero-copy exchange (DLPack) ------------------------------------------
+    def __dlpack__(self, stream=None, max_version=None, dl_device=None, copy=None):
+        """The amplitudes as a DLPack capsule sharing this state's device
+        memory (``torch.from_dlpack(sv)``). Queued gates run and are waited
+        for first, so the data is complete on every stream; later gates show
+        in the consumer's tensor, which keeps the memory alive after this
+        object is gone."""
+        if copy:
+            raise BufferError("the state is exported without a copy (copy=True is not offered)")
+        p = C.c_void_p()
+        check(lib.lq_sv_dlpack(self._h, C.byref(p)))
+        return _dlpack_capsule(p.value)
+
+    def __dlpack_device__(self):
+        p = C.c_void_p()
+        check(lib.lq_sv_dlpack(self._h, C.byref(p)))
+        dev = _DLManagedTensor.from_address(p.value).dl_tensor.device
+        out = (int(dev.device_type), int(dev.device_id))
+        check(lib.lq_dlpack_release(p))
+        return out
+
+    def to_torch(self):
+        """A torch tensor on the GPU viewing this state's amplitudes (no copy)."""
+        import torch
+        return torch.from_dlpack(self)
 
     def device_ptr(self) -> int:
         p = C.c_void_p()

@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import lightning_qubit_ops as L
+from .adjoint_jacobian import cotangent_observable, observable
 from .workloads import CircuitOp, apply_circuit, to_ops
 
 
@@ -61,40 +62,6 @@ def num_params(ops) -> int:
     return sum(len(op.params) for op in ops)
 
 
-def _obs(spec, bits):
-    sfx = f"C{bits}"
-    tag = spec[0]
-    if tag == "N":
-        return getattr(L, "NamedObs" + sfx)(spec[1], [int(spec[2])])
-    if tag == "T":
-        return getattr(L, "TensorProdObs" + sfx)([getattr(L, "NamedObs" + sfx)(nm, [int(w)]) for nm, w in spec[1]])
-    if tag == "M":
-        return getattr(L, "HermitianObs" + sfx)(np.asarray(spec[1]), [int(w) for w in spec[2]])
-    if tag == "H":
-        return getattr(L, "Hamiltonian" + sfx)([float(c) for c in spec[1]], [_obs(t, bits) for t in spec[2]])
-    raise ValueError(f"bad observable {spec!r}")
-
-
-def _weighted_terms(spec, w):
-    """Flatten w * spec into (coefficient, term) pairs of non-Hamiltonian terms."""
-    if spec[0] == "H":
-        out = []
-        for c, t in zip(spec[1], spec[2]):
-            out += _weighted_terms(t, w * float(c))
-        return out
-    return [(w, spec)]
-
-
-def _vjp_observable(observables, dy):
-    terms = []
-    for spec, w in zip(observables, dy):
-        if w != 0.0:
-            terms += _weighted_terms(spec, float(w))
-    if not terms:
-        return None
-    return ("H", [c for c, _ in terms], [t for _, t in terms])
-
-
 class _CircuitExpvals(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params, n, ops, observables, bits):
@@ -103,7 +70,7 @@ class _CircuitExpvals(torch.autograd.Function):
         sv = (L.StateVectorC128 if bits == 128 else L.StateVectorC64)(n)
         apply_circuit(sv, bound)
         m = (L.MeasurementsC128 if bits == 128 else L.MeasurementsC64)(sv)
-        ev = np.array([m.expval(_obs(o, bits)) for o in observables], dtype=np.float64)
+        ev = np.array([m.expval(observable(o, bits)) for o in observables], dtype=np.float64)
         ctx.sv, ctx.bound, ctx.observables, ctx.bits = sv, bound, observables, bits
         ctx.param_dtype, ctx.param_device = params.dtype, params.device
         return torch.as_tensor(ev, dtype=params.dtype, device=params.device)
@@ -113,11 +80,11 @@ class _CircuitExpvals(torch.autograd.Function):
         dy = grad_out.detach().to("cpu", torch.float64).numpy().reshape(-1)
         npar = num_params(ctx.bound)
         grad = np.zeros(npar, dtype=np.float64)
-        ham = _vjp_observable(ctx.observables, dy)
+        ham = cotangent_observable(ctx.observables, dy)
         if ham is not None and npar:
             adj = (L.algorithms.AdjointJacobianC128 if ctx.bits == 128 else L.algorithms.AdjointJacobianC64)()
-            grad = np.asarray(adj(ctx.sv, [_obs(ham, ctx.bits)], to_ops(ctx.bound, ctx.bits), list(range(npar))),
-                              dtype=np.float64)
+            grad = np.asarray(adj(ctx.sv, [observable(ham, ctx.bits)], to_ops(ctx.bound, ctx.bits),
+                                  list(range(npar))), dtype=np.float64)
         ctx.sv.release_workspace()
         return (torch.as_tensor(grad, dtype=ctx.param_dtype, device=ctx.param_device), None, None, None, None)
 

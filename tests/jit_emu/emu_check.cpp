@@ -9,6 +9,9 @@
 //   CASE 1: the same plus controlled, parity, phase and SWAP gates
 //   CASE 2: case 1 on two states with dots between them (adjoint-like)
 //   CASE 3: hardware-efficient layers, then four named expectation values
+//   CASE 4: hardware-efficient layers on three states with dots of the first
+//           against each of the others (observables sharing an adjoint sweep)
+//   CASE 5: the same on four states
 // Exit status 0 when every amplitude and dot matches (complex128 1e-10,
 // complex64 1e-3 relative).
 #include <dlfcn.h>
@@ -85,7 +88,8 @@ static std::string transform(std::string s) {
     const unsigned long lds_bytes = lp == std::string::npos ? 0 : std::stoul(s.substr(lp + 13));
     rep("extern __shared__ cplx lds[];",
         "static LdsArr<cplx, " + std::to_string(lds_bytes) + "ul / sizeof(cplx)> lds;");
-    rep("__shared__ double2 dred[4][32];", "static double2 dred[4][32];");
+    for (int d = 32; d <= 96; d += 32)
+        rep("__shared__ double2 dred[4][" + std::to_string(d) + "];", "static double2 dred[4][" + std::to_string(d) + "];");
     rep("asm volatile(\"\" : \"+s\"(z));", "");
     rep("typedef __attribute__((address_space(4))) const double2 kcoef;", "typedef const double2 kcoef;");
     rep("typedef __attribute__((address_space(4))) const unsigned long long kgeo;", "typedef const unsigned long long kgeo;");
@@ -93,7 +97,7 @@ static std::string transform(std::string s) {
     return "#define LQ_EMU 1\n#include \"emu_rt.hpp\"\n" + s;
 }
 
-using Launch = void (*)(void *, void *, const double2 *, double2 *, unsigned long long, unsigned);
+using Launch = void (*)(void *, void *, const double2 *, double2 *, unsigned long long, unsigned, void *, void *);
 
 int main(int argc, char **argv) {
     const int which = atoi(argv[1]), dtype = atoi(argv[2]);
@@ -118,6 +122,8 @@ int main(int argc, char **argv) {
         prog.push_back(d);
     };
     if (which == 2) ns = 2;
+    if (which == 4) ns = 3; // lambda and two H|lambda> (observables sharing an adjoint sweep)
+    if (which == 5) ns = 4;
     if (which == 3) {
         // hardware-efficient layers, then <Z_0>, <X_{n-1}>, <Y_8>, <H_3> as the
         // engine lowers named expectation values (one dot each)
@@ -136,7 +142,7 @@ int main(int argc, char **argv) {
     for (int l = 0; l < (which == 3 ? 0 : 3); ++l) {
         for (int w = 0; w < n; ++w) gate(GateId::Rot, {(uint32_t)w}, {U(rng), U(rng), U(rng)});
         for (int s = 0; s < 2; ++s) for (int w = s; w < n - 1; w += 2) gate(GateId::CZ, {(uint32_t)w, (uint32_t)w + 1}, {});
-        if (which >= 1) {
+        if (which >= 1 && which != 4 && which != 5) {
             gate(GateId::CRY, {1, 7}, {U(rng)});
             gate(GateId::IsingZZ, {0, (uint32_t)n - 1}, {U(rng)});
             gate(GateId::RX, {3}, {U(rng)}, {9}, {0});
@@ -145,17 +151,26 @@ int main(int argc, char **argv) {
             gate(GateId::PhaseShift, {4}, {U(rng)});
             gate(GateId::SWAP, {6, 12}, {});
         }
-        if (which == 2 || l == 2) {
-            dot(GateId::RX, (uint32_t)n - 1, 0, ns - 1);
-            dot(GateId::RZ, 0, 0, ns - 1);
-            dot(GateId::RY, 5, 0, ns - 1);
+        if (which >= 4) {
+            // generator dots of every wire's rotation against each observable state, as the
+            // sweep of several observables makes them (passes fill more than 32 slots)
+            for (int w = 0; w < n; ++w)
+                for (int sb = 1; sb < ns; ++sb) dot(w % 2 ? GateId::RY : GateId::RZ, (uint32_t)w, 0, sb);
+        }
+        if (which == 2 || which >= 4 || l == 2) {
+            for (int sb = std::min(1, ns - 1); sb < ns; ++sb) {
+                dot(GateId::RX, (uint32_t)n - 1, 0, sb);
+                dot(GateId::RZ, 0, 0, sb);
+                dot(GateId::RY, 5, 0, sb);
+            }
         }
     }
-    PlanConfig cfg; cfg.n = n; cfg.ns = ns; cfg.kt_max = 12; cfg.low_bits = getenv("EMU_LOW_BITS") ? atoi(getenv("EMU_LOW_BITS")) : (getenv("EMU_NO_SPLIT") ? (dtype ? 4 : 5) : 4); cfg.reg_rb = 4; cfg.max_dots = 32; cfg.scale_rotations = !getenv("EMU_NO_SCALE");
+    PlanConfig cfg; cfg.n = n; cfg.ns = ns; cfg.kt_max = ns > 2 ? 11 : 12; cfg.low_bits = getenv("EMU_LOW_BITS") ? atoi(getenv("EMU_LOW_BITS")) : (getenv("EMU_NO_SPLIT") ? (dtype ? 4 : 5) : 4); cfg.reg_rb = ns > 2 ? 3 : 4; cfg.max_dots = ns > 2 ? std::min(kMaxJitPassDots, kMaxPassDots * (ns - 1)) : kMaxPassDots; cfg.scale_rotations = !getenv("EMU_NO_SCALE");
     cfg.max_scale_growth = dtype ? 1e300 : 1e20; // as the engine configures specialised passes
-    // the engine's specialised-pass planning: lookahead tile bits, budget 52, 4 low bits
+    // the engine's specialised-pass planning: lookahead tile bits, budget 52 (one state; 78 for two,
+    // 26 (ns + 1) for more), 4 low bits
     cfg.lookahead = !(getenv("EMU_LOOKAHEAD") && atoi(getenv("EMU_LOOKAHEAD")) == 0);
-    cfg.budget = getenv("EMU_NO_SPLIT") ? 24 : 52;
+    cfg.budget = getenv("EMU_NO_SPLIT") ? 24 : (ns == 1 ? 52 : ns == 2 ? 78 : 26.0 * (ns + 1));
     // EMU_BASIS=k (one state): the state is the basis state |k> not yet
     // written (a deferred reset); as in the engine, the program's leading
     // one-qubit gates are folded into its product factors and the first pass
@@ -185,7 +200,7 @@ int main(int argc, char **argv) {
     for (const auto &o : prog) ref_apply(R, n, o, rdots);
     // emulated run
     const size_t esz = dtype ? 16 : 8;
-    std::vector<std::vector<char>> buf(2, std::vector<char>(esz << n));
+    std::vector<std::vector<char>> buf(std::max(2, ns), std::vector<char>(esz << n));
     for (int s = 0; s < ns; ++s)
         for (uint64_t i = 0; i < (1ull << n); ++i) {
             // with a deferred reset the memory holds unrelated values
@@ -228,7 +243,8 @@ int main(int argc, char **argv) {
         const unsigned grid = (unsigned)ntiles;
         (void)grid_cap;
         auto t0 = std::chrono::steady_clock::now();
-        L(buf[0].data(), ns > 1 ? buf[1].data() : nullptr, coef.data(), part.data(), ntiles, grid);
+        L(buf[0].data(), ns > 1 ? buf[1].data() : nullptr, coef.data(), part.data(), ntiles, grid,
+          ns > 2 ? buf[2].data() : nullptr, ns > 3 ? buf[3].data() : nullptr);
         double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for (size_t sl = 0; sl < passes[p].slot_ids.size(); ++sl) {
             cd sum = 0;

@@ -14,9 +14,9 @@ double __shfl_xor(double v, int off, int) {
     g_bar->arrive_and_wait();
     return r;
 }
-extern "C" void lqpass(void *, void *, const double2 *, double2 *, unsigned long long);
+extern "C" void lqpass(void *, void *, const double2 *, double2 *, unsigned long long, void *, void *);
 extern "C" void emu_launch(void *sv0, void *sv1, const double2 *coef, double2 *partials, unsigned long long ntiles,
-                           unsigned grid) {
+                           unsigned grid, void *sv2, void *sv3) {
     gridDim = {grid, 1, 1};
     for (unsigned b = 0; b < grid; ++b) {
         blockIdx = {b, 0, 0};
@@ -26,7 +26,7 @@ extern "C" void emu_launch(void *sv0, void *sv1, const double2 *coef, double2 *p
         for (unsigned t = 0; t < 256; ++t)
             th.emplace_back([=] {
                 threadIdx = {t, 0, 0};
-                lqpass(sv0, sv1, coef, partials, ntiles);
+                lqpass(sv0, sv1, coef, partials, ntiles, sv2, sv3);
             });
         for (auto &x : th) x.join();
     }

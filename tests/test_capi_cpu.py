@@ -42,3 +42,18 @@ def test_python_mirror_surface():
     for name in ("OpsStructC128", "AdjointJacobianC128", "create_ops_listC128", "OpsStructC64",
                  "AdjointJacobianC64", "create_ops_listC64"):
         assert hasattr(L.algorithms, name)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="liblq_amd.so not built")
+def test_one_hip_runtime_with_torch():
+    """The binding imports torch before the engine, so the engine binds to the
+    HIP runtime libraries torch bundles (same sonames): one runtime per process."""
+    import subprocess
+    import sys
+    code = ("import re, lightning_amd._capi\n"
+            "m = {l.split()[-1] for l in open('/proc/self/maps') if re.search(r'libamdhip64|libhsa-runtime64', l)}\n"
+            "print(len([x for x in m if 'amdhip64' in x]), len([x for x in m if 'hsa-runtime64' in x]))\n")
+    env = dict(os.environ, PYTHONPATH=os.path.join(ROOT, "pennylane-lightning_amd"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.split() == ["1", "1"], r.stdout

@@ -128,3 +128,40 @@ def test_jit_adjoint(bits):
         return np.asarray(jac).reshape(2, -1), sv
 
     run_twice(make, lambda j: np.testing.assert_allclose(j, jexp, atol=tol))
+
+
+@pytest.mark.parametrize("bits", [128, 64])
+def test_adjoint_observables_share_a_sweep(bits):
+    """Four observables: once their three-observable program's specialised
+    kernels are ready, three share one backward sweep (lambda and three
+    H|lambda> in four-state passes) and the fourth runs alone; before that
+    all four run one sweep each. Both against the oracle
+    (lq/algorithms/AdjointJacobianLQubit.hpp:347-491, one row per observable)."""
+    n = 16
+    ops = W.strongly_entangling(n, 1, seed=12)
+    ops += [W.CircuitOp("CRX", [3, 11], [0.4]), W.CircuitOp("IsingYY", [1, 14], [0.7]),
+            W.CircuitOp("PhaseShift", [6], [0.9], inverse=True)]
+    coeffs, words = W.random_pauli_hamiltonian(n, 10, seed=4, law=(0.5, 0.2, 0.15, 0.15))
+    npar = sum(1 for op in ops if op.params)
+    tp = list(range(1, npar, 3))
+    st = oracle_run(n, ops)
+    named = [("PauliZ", 4), ("PauliX", 9), ("PauliY", 2)]
+    oobs = [ham_to_oracle(coeffs, words)] + [("N", nm, w) for nm, w in named]
+    jexp = O.adjoint_jacobian(st, n, to_oracle_ops(ops), oobs, tp)
+    eexp = [O.expval_obs(st, n, o) for o in oobs]
+    sfx = f"C{bits}"
+    tol = 1e-10 if bits == 128 else 5e-4
+
+    def make():
+        sv = SV[bits](n)
+        W.apply_circuit(sv, ops)
+        obs = [W.hamiltonian_obs(coeffs, words, bits)] + [getattr(L, "NamedObs" + sfx)(nm, [w]) for nm, w in named]
+        adj = getattr(L.algorithms, "AdjointJacobian" + sfx)()
+        jac, ev = adj(sv, obs, W.to_ops(ops, bits), tp, return_expvals=True)
+        return (np.asarray(jac).reshape(len(obs), -1), np.asarray(ev)), sv
+
+    def check(r):
+        np.testing.assert_allclose(r[0], jexp, atol=tol)
+        np.testing.assert_allclose(r[1], eexp, atol=tol)
+
+    run_twice(make, check)

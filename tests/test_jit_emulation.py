@@ -1,8 +1,9 @@
 """Generated pass kernels run on the CPU under emulation (tests/jit_emu):
 the engine's planner and kernel generator in liblq_amd.so produce each pass
 of three programs (hardware-efficient layers with dots; plus controlled,
-parity, phase and SWAP gates; the same on two states with dots between them,
-as in the adjoint sweep; and layers followed by named expectation values),
+parity, phase and SWAP gates; the same on two, three and four states with dots between them,
+as in the adjoint sweep with one to three observables; and layers followed by
+named expectation values),
 the kernels are compiled for the host, run with one
 host thread per GPU thread, and compared with a naive application of the same
 ops (complex128 within 1e-10, complex64 within 1e-3 relative)."""
@@ -39,7 +40,8 @@ def emu_binary(tmp_path_factory):
     (0, 1, 1, 1, 14), (2, 1, 1, 1, 14), (2, 0, 1, 1, 14), (0, 1, 0, 0, 14), (1, 0, 0, 0, 14), (2, 1, 0, 0, 14),
     (0, 1, 1, 2, 16), (1, 0, 1, 2, 16), (2, 1, 1, 2, 16), (0, 1, 1, 3, 14), (1, 0, 1, 3, 14),
     (3, 0, 1, 1, 17), (3, 0, 1, 3, 17), (3, 1, 1, 3, 17), (2, 1, 1, 4, 14), (1, 0, 1, 4, 14), (0, 1, 1, 5, 14),
-    (0, 1, 1, 6, 14), (2, 1, 1, 6, 14), (1, 0, 1, 6, 14), (1, 0, 1, 7, 14), (3, 0, 1, 7, 17)])
+    (0, 1, 1, 6, 14), (2, 1, 1, 6, 14), (1, 0, 1, 6, 14), (1, 0, 1, 7, 14), (3, 0, 1, 7, 17),
+    (4, 1, 1, 1, 14), (4, 0, 1, 1, 14), (5, 1, 1, 1, 15), (5, 0, 1, 1, 14)])
 def test_generated_kernels_match_naive_application(emu_binary, case, dtype, lds32, look, n):
     exe, work = emu_binary
     # lds32: transposes through 32 KiB of LDS (emulated at exactly the size the kernel declares);
@@ -49,6 +51,7 @@ def test_generated_kernels_match_naive_application(emu_binary, case, dtype, lds3
     # look 3: lookahead, and the state a basis state not yet written (deferred reset): the leading
     # one-qubit gates are folded into product factors and the first pass forms the product state
     # case 3 at 17 qubits and complex64: the program of the complex64 expval GPU test
+    # cases 4 and 5: three and four states (the adjoint sweep with two and three observables at once)
     env = dict(os.environ, LQ_JIT_LDS32=str(lds32), EMU_LOOKAHEAD=str(min(look, 1)),
                LQ_JIT_DIRECT="2" if look == 2 else "1")
     # look 4: bit positions read from the table after the coefficients (LQ_JIT_LIFT); look 5: that and
