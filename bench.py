@@ -32,6 +32,7 @@ change -- Rot: all (2 x 16 GiB), CZ: the |11> quarter (2 x 4 GiB).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -50,7 +51,7 @@ HBM_PEAK = 8.0e12
 # WRITE_SIZE in separate runs of this script, each scaled by the same counter
 # over the streaming copy kernel of a known size): (precision, qubits) ->
 # (bytes, profile summary holding the runs)
-MEASURED_TRAFFIC = {("c128", 30): (32453777030, "profiles/lqpass_hea30_c128_traffic.md")}
+MEASURED_TRAFFIC = {("c128", 30): (32453964496, "profiles/lqpass_hea30_c128_traffic.md")}
 
 
 def gate_bytes(op, n, elem):
@@ -253,6 +254,59 @@ def run_adjoint_multi(args):
                       "steps": args.steps}))
 
 
+def run_graph(args):
+    """Circuit replays from a HIP graph against eager application: the
+    hardware-efficient ansatz in adjoint form (per layer and wire RZ RY RZ,
+    then CZ on even and odd pairs), 10 layers, complex128, at 12, 16 and 20
+    qubits. Eager: reset + applyOperations + synchronize; replay: new
+    parameters into the recorded graph + synchronize. Each figure is the mean
+    over args.steps circuits with fresh angles."""
+    from lightning_amd import lightning_qubit_ops as L
+    from lightning_amd import workloads as W
+
+    rng = np.random.default_rng(0)
+    rows = []
+    for n in (12, 16, 20):
+        ops = W.hea_rot_cz_adjoint_form(n, 10, seed=1)
+        npar = sum(len(o.params) for o in ops)
+        sv = L.StateVectorC128(n)
+        rec = sv.record(W.to_ops(ops, 128))
+        sets = [rng.uniform(0, 2 * math.pi, npar) for _ in range(args.steps)]
+
+        def bind(vals):
+            it = iter(vals)
+            return [W.CircuitOp(o.name, o.wires, [next(it) for _ in o.params], o.inverse, o.matrix, o.ctrls, o.cvals)
+                    for o in ops]
+
+        bound = [bind(v) for v in sets]
+        for _ in range(max(args.warmup, 1)):
+            sv.resetStateVector()
+            W.apply_circuit(sv, bound[0])
+            sv.synchronize()
+            rec.replay(sets[0])
+            sv.synchronize()
+        L.jit_wait()
+        rec.replay(sets[0])
+        sv.resetStateVector()
+        W.apply_circuit(sv, bound[0])
+        sv.synchronize()
+        t0 = time.perf_counter()
+        for b in bound:
+            sv.resetStateVector()
+            W.apply_circuit(sv, b)
+            sv.synchronize()
+        eager = (time.perf_counter() - t0) * 1e3 / len(bound)
+        t0 = time.perf_counter()
+        for v in sets:
+            rec.replay(v)
+            sv.synchronize()
+        replay = (time.perf_counter() - t0) * 1e3 / len(sets)
+        rows.append({"n_qubits": n, "gates": len(ops), "eager_ms": round(eager, 4), "replay_ms": round(replay, 4),
+                     "speedup": round(eager / replay, 3), **rec.stats()})
+    print(json.dumps({"metric": "circuit time, HIP-graph replay vs eager (HEA 10 layers, complex128)", "rows": rows,
+                      "n_gpus": 1, "dtype": "complex128", "data": "synthetic", "steps": args.steps}))
+
+
 def serving_sweep(count, qps_batched, qps_plain, bits=128, seed=0, server_kw=None):
     """Replay `count` requests at each arrival rate through the batched
     server, then through the plain path; returns the summary object."""
@@ -290,6 +344,8 @@ def serving_sweep(count, qps_batched, qps_plain, bits=128, seed=0, server_kw=Non
         return r.get("steady_req_per_s", r["req_per_s"])
 
     plain = [r for r in rows if r["mode"] == "plain"]
+    if not plain or not qps_batched:  # one mode only (profiling runs): no ratio
+        return {"ratio_at_equal_p99": None, "requests": count, "sweep": rows, "setup_s": round(setup_s, 2)}
     plain_ok = [r for r in plain if keeps_up(r)] or [min(plain, key=lambda r: r["p99_ms"])]
     best_plain = max(plain_ok, key=rate)
     ok = [r for r in rows if r["mode"] == "batched" and keeps_up(r) and r["p99_ms"] <= best_plain["p99_ms"]]
@@ -307,8 +363,12 @@ def run_serving(args):
     Hamiltonian; each asks for the energy and its full adjoint gradient
     (complex128). Poisson arrivals at each rate of a sweep, through the
     batched server and through the plain one-request-at-a-time path."""
-    qps_plain = [float(x) for x in (args.qps or args.sweep_plain or "40,60,80,100,120,140").split(",")]
-    qps_batched = [float(x) for x in (args.qps or args.sweep_batched or "80,120,160,200,240,300").split(",")]
+    def rates(spec, default):
+        spec = args.qps or spec or default
+        return [] if spec == "none" else [float(x) for x in spec.split(",")]
+
+    qps_plain = rates(args.sweep_plain, "40,60,80,100,120,140")
+    qps_batched = rates(args.sweep_batched, "80,120,160,200,240,300")
     kw = {}
     if args.serve_small_max is not None:
         kw["small_max_qubits"] = args.serve_small_max
@@ -534,7 +594,7 @@ def main():
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the plain streaming kernel (4 GiB read + 4 GiB written per launch)")
     ap.add_argument("--workload", choices=["hea30", "adjoint24", "serving", "qft33", "precision", "mfma5",
-                                           "adjoint_multi"],
+                                           "adjoint_multi", "graph"],
                     default="hea30",
                     help="hea30: the gate-application line (default); adjoint24: the 24-qubit adjoint gradient; "
                          "serving: energy + gradient requests, batched vs plain; qft33: 33-qubit QFT + Haar "
@@ -546,8 +606,9 @@ def main():
     ap.add_argument("--serving-qps-batched", default="300,450,550")
     ap.add_argument("--serving-qps-plain", default="120,170,220")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
-    ap.add_argument("--sweep-batched", default="", help="serving workload: arrival rates of the batched server")
-    ap.add_argument("--sweep-plain", default="", help="serving workload: arrival rates of the plain path")
+    ap.add_argument("--sweep-batched", default="",
+                    help="serving workload: arrival rates of the batched server ('none': skip that mode)")
+    ap.add_argument("--sweep-plain", default="", help="serving workload: arrival rates of the plain path ('none': skip)")
     ap.add_argument("--serve-small-max", type=int, default=None,
                     help="serving: largest qubit count run in the packed kernel (default: the server's)")
     ap.add_argument("--serve-workers", type=int, default=None,
@@ -582,6 +643,8 @@ def main():
         return run_mfma5(args)
     if args.workload == "adjoint_multi":
         return run_adjoint_multi(args)
+    if args.workload == "graph":
+        return run_graph(args)
     from lightning_amd import lightning_qubit_ops as L
     from lightning_amd import workloads as W
 

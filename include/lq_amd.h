@@ -85,6 +85,28 @@ int lq_sv_get_data_device(lq_sv_t sv, void *dst, uint64_t len);
 int lq_sv_dlpack(lq_sv_t sv, void **out);
 int lq_dlpack_release(void *managed);
 
+/* Recorded circuits (HIP graphs; no counterpart in the reference, whose
+ * gates run eagerly on the CPU): lq_sv_record prepares |0...0>, applies
+ * `ops` and keeps that program as a HIP graph captured on the state's
+ * stream (program upload, state preparation, every fused pass).
+ * lq_rec_replay runs the same circuit again from |0...0> with new
+ * parameters: `params` holds every op's parameters in op order (nparams =
+ * the recorded ops' total, including the 0.0 placeholders of matrix ops).
+ * A replay rewrites the pass coefficients and launches the graph; when the
+ * new parameters plan differently (specialised kernels compiled since the
+ * capture, a rotation that changes form) the circuit is captured again
+ * (lq_rec_stats counts captures and replays). Queued gates of the state are
+ * discarded by both calls. The recording keeps the state alive; replays of
+ * one recording are serialised on the state's stream; the state is not
+ * synchronised on return (measurements and lq_sv_synchronize order after
+ * it). Intended for parameter loops over one circuit structure (variational
+ * training, serving repeated request shapes). */
+typedef struct lq_rec_s *lq_rec_t;
+int lq_sv_record(lq_sv_t sv, lq_ops_t ops, lq_rec_t *out);
+int lq_rec_replay(lq_rec_t rec, const double *params, uint64_t nparams);
+int lq_rec_stats(lq_rec_t rec, uint64_t *captures, uint64_t *replays);
+int lq_rec_destroy(lq_rec_t rec);
+
 /* applyOperation(name, [controlled_wires, controlled_values,] wires, inverse, params)
  * (lq/StateVectorLQubit.hpp:378-423; binding core/bindings/Bindings.hpp:213-277).
  * nc = 0 for the plain form. "Identity" is a no-op. */

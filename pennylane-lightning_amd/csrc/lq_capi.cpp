@@ -24,6 +24,10 @@ struct lq_sv_s {
 struct lq_obs_s {
     std::shared_ptr<Obs> obs;
 };
+struct lq_rec_s {
+    std::shared_ptr<StateVector> sv; // the graph writes this state
+    std::unique_ptr<StateVector::Recording> rec;
+};
 struct lq_ops_s {
     OpsList list;
 };
@@ -150,6 +154,41 @@ int lq_sv_device_ptr(lq_sv_t sv, void **out) {
         // data() first: it may enqueue the write of a deferred reset
         *out = S(sv).data();
         S(sv).synchronize();
+    });
+}
+
+int lq_sv_record(lq_sv_t sv, lq_ops_t ops, lq_rec_t *out) {
+    return guard([&] {
+        LQ_ABORT_IF(ops == nullptr || out == nullptr, "null argument");
+        StateVector &s = S(sv);
+        auto h = std::make_unique<lq_rec_s>();
+        h->sv = sv->sv;
+        h->rec = s.record(ops->list);
+        *out = h.release();
+    });
+}
+
+int lq_rec_replay(lq_rec_t rec, const double *params, uint64_t nparams) {
+    return guard([&] {
+        LQ_ABORT_IF(rec == nullptr || !rec->rec, "null recording handle");
+        rec->sv->replay(*rec->rec, params, static_cast<size_t>(nparams));
+    });
+}
+
+int lq_rec_stats(lq_rec_t rec, uint64_t *captures, uint64_t *replays) {
+    return guard([&] {
+        LQ_ABORT_IF(rec == nullptr || !rec->rec, "null recording handle");
+        if (captures) *captures = rec->rec->captures;
+        if (replays) *replays = rec->rec->replays;
+    });
+}
+
+int lq_rec_destroy(lq_rec_t rec) {
+    return guard([&] {
+        if (rec == nullptr) return;
+        // the graph may still run on the state's stream
+        if (rec->sv) (void)hipStreamSynchronize(rec->sv->stream());
+        delete rec;
     });
 }
 

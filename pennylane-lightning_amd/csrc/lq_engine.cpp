@@ -775,10 +775,15 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         }
     }
 
+    // a capture (record) stages into the recording's own buffers, which its
+    // graph reads at every replay
+    HostPinned &stg = capturing_ ? capturing_->staging : staging_;
+    DevBuf &pbuf = capturing_ ? capturing_->prog : prog_;
+    LQ_ABORT_IF(capturing_ && part_n != 0, "a recorded circuit carries no dots");
     // the staging buffer may still feed the previous upload
-    hip_check(hipEventSynchronize(staged_), "hipEventSynchronize");
-    staging_.ensure(total);
-    char *hs = static_cast<char *>(staging_.p);
+    if (!capturing_) hip_check(hipEventSynchronize(staged_), "hipEventSynchronize");
+    stg.ensure(total);
+    char *hs = static_cast<char *>(stg.p);
     for (size_t p = 0; p < passes.size(); ++p) {
         std::memcpy(hs + op_off[p], passes[p].ops.data(), passes[p].ops.size() * sizeof(TileOp));
         if (!passes[p].stages.empty())
@@ -799,13 +804,17 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
     }
     if (form_basis) prep_pending_ = false;
     if (!ranges.empty()) std::memcpy(hs + ranges_at, ranges.data(), ranges.size() * sizeof(SlotRange));
-    prog_.ensure(total);
-    hip_check(hipMemcpyAsync(prog_.p, hs, total, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
-    hip_check(hipEventRecord(staged_, stream_), "hipEventRecord");
+    pbuf.ensure(total);
+    hip_check(hipMemcpyAsync(pbuf.p, hs, total, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
+    if (!capturing_) hip_check(hipEventRecord(staged_, stream_), "hipEventRecord");
     if (part_n) partials_.ensure(part_n * sizeof(double2));
+    if (capturing_) {
+        capturing_->coef_at = coef_at;
+        capturing_->coef_off = coef_off;
+    }
+    const bool prof = prof_ && !capturing_;
 
-
-    char *dp = static_cast<char *>(prog_.p);
+    char *dp = static_cast<char *>(pbuf.p);
     for (size_t p = 0; p < passes.size(); ++p) {
         const PassPlan &pp = passes[p];
         TilePassArgs a{};
@@ -844,7 +853,7 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
             ra.g = a.g;
         }
         Pending pe{};
-        if (prof_) {
+        if (prof) {
             pe.a = take_event();
             pe.b = take_event();
             // a pass that forms a deferred reset reads nothing of that state
@@ -864,7 +873,7 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         else if (pp.reg) dispatch(dtype_, [&](auto tt) { launch_tile_reg<decltype(tt)>(ra, lds, stream_); });
         else dispatch(dtype_, [&](auto tt) { launch_tile_pass<decltype(tt)>(a, lds, stream_); });
         ++(fns[p] ? jit_passes_ : generic_passes_);
-        if (prof_) {
+        if (prof) {
             hip_check(hipEventRecord(pe.b, stream_), "hipEventRecord");
             pending_.push_back(pe);
         }
@@ -882,6 +891,117 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         for (int i = 0; i < nslots; ++i) result[i] = cd(h[i].x, h[i].y);
     }
     return result;
+}
+
+// ---------------------------------------------------------------------------
+// Recorded circuits (HIP graphs)
+
+StateVector::Recording::~Recording() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+}
+
+std::vector<LOp> StateVector::lower_recorded(const Recording &r) {
+    queue_.clear();
+    for (const OpRec &op : r.ops) {
+        if (op.named && op.gid == GateId::Identity) continue;
+        apply_oprec(op, false); // checks the op as applyOperation does, then queues it
+    }
+    std::vector<LOp> prog;
+    prog.swap(queue_);
+    return prog;
+}
+
+void StateVector::capture(Recording &r, Planned pl) {
+    if (r.exec) (void)hipGraphExecDestroy(r.exec);
+    if (r.graph) (void)hipGraphDestroy(r.graph);
+    r.exec = nullptr;
+    r.graph = nullptr;
+    // the previous launch of this recording may still read its buffers
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    set_basis_index(0);
+    // relaxed: the capture sizes the recording's own buffers (hipHostMalloc,
+    // hipMalloc), which the stricter capture modes refuse
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    capturing_ = &r;
+    try {
+        (void)execute(pl, {d_}, 0);
+    } catch (...) {
+        capturing_ = nullptr;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    capturing_ = nullptr;
+    hip_check(hipStreamEndCapture(stream_, &r.graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&r.exec, r.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    r.planned = std::move(pl);
+    ++r.captures;
+    // capturing ran nothing: this launch prepares the state
+    hip_check(hipGraphLaunch(r.exec, stream_), "hipGraphLaunch");
+    queue_.clear();
+    prep_pending_ = false;
+}
+
+std::unique_ptr<StateVector::Recording> StateVector::record(const OpsList &ops) {
+    auto r = std::make_unique<Recording>();
+    r->owner = this;
+    r->ops = ops.ops;
+    for (const OpRec &op : r->ops) r->nparams += op.params.size();
+    LQ_ABORT_IF(r->ops.size() > 8000, "too many operations to record in one graph");
+    std::vector<LOp> prog = lower_recorded(*r);
+    LQ_ABORT_IF(prog.empty(), "nothing to record");
+    set_basis_index(0);
+    capture(*r, plan(prog, 1, true));
+    return r;
+}
+
+namespace {
+// the same kernels, pass layout and coefficient counts: only coefficient
+// values differ
+bool same_shape(const StateVector::Planned &a, const StateVector::Planned &b) {
+    if (a.passes.size() != b.passes.size() || a.fns != b.fns || a.geo != b.geo) return false;
+    for (size_t p = 0; p < a.passes.size(); ++p) {
+        const PassPlan &x = a.passes[p], &y = b.passes[p];
+        if (x.reg != y.reg || x.tbits != y.tbits || x.coef.size() != y.coef.size() || x.load_mask != y.load_mask ||
+            x.store_mask != y.store_mask || x.end_transpose != y.end_transpose || x.ops.size() != y.ops.size() ||
+            x.stages.size() != y.stages.size() || x.slot_ids != y.slot_ids)
+            return false;
+        if (!x.ops.empty() && std::memcmp(x.ops.data(), y.ops.data(), x.ops.size() * sizeof(TileOp)) != 0) return false;
+        if (!x.stages.empty() && std::memcmp(x.stages.data(), y.stages.data(), x.stages.size() * sizeof(TileStage)) != 0)
+            return false;
+    }
+    return true;
+}
+} // namespace
+
+void StateVector::replay(Recording &r, const double *params, size_t np) {
+    LQ_ABORT_IF(r.owner != this, "the recording belongs to another state vector");
+    LQ_ABORT_IF(np != r.nparams, "wrong number of parameters for the recorded circuit");
+    LQ_ABORT_IF(np > 0 && params == nullptr, "null argument");
+    size_t k = 0;
+    for (OpRec &op : r.ops)
+        for (double &v : op.params) v = params[k++];
+    std::vector<LOp> prog = lower_recorded(r);
+    set_basis_index(0);
+    Planned pl = plan(prog, 1, true);
+    if (!same_shape(pl, r.planned)) {
+        capture(r, std::move(pl));
+        return;
+    }
+    // the previous launch has read the staging buffer once this returns
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    char *hs = static_cast<char *>(r.staging.p);
+    for (size_t p = 0; p < pl.passes.size(); ++p) {
+        double2 *c = reinterpret_cast<double2 *>(hs + r.coef_at) + r.coef_off[p];
+        const std::vector<cd> &cf = pl.passes[p].coef;
+        for (size_t i = 0; i < cf.size(); ++i) c[i] = make_double2(cf[i].real(), cf[i].imag());
+    }
+    hip_check(hipGraphLaunch(r.exec, stream_), "hipGraphLaunch");
+    ++r.replays;
+    queue_.clear();
+    prep_pending_ = false;
 }
 
 // ---------------------------------------------------------------------------

@@ -232,6 +232,32 @@ class StateVector {
     Planned plan(const std::vector<LOp> &prog, int ns, bool forms = false);
     std::vector<cd> execute(const Planned &pl, const std::vector<void *> &states, int nslots);
 
+    // A recorded circuit: |0...0> followed by `ops`, captured once on this
+    // state's stream as a HIP graph (program upload, the state's preparation,
+    // every pass). A replay writes the pass coefficients of new parameters
+    // into the graph's pinned staging buffer and launches the graph; when the
+    // plan of the new parameters differs (kernels compiled since, a rotation
+    // that changes form) the circuit is captured again.
+    struct Recording {
+        StateVector *owner = nullptr;
+        std::vector<OpRec> ops;
+        size_t nparams = 0;
+        Planned planned;
+        size_t coef_at = 0;
+        std::vector<size_t> coef_off;
+        HostPinned staging;
+        DevBuf prog;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        uint64_t captures = 0, replays = 0;
+        Recording() = default;
+        Recording(const Recording &) = delete;
+        Recording &operator=(const Recording &) = delete;
+        ~Recording();
+    };
+    std::unique_ptr<Recording> record(const OpsList &ops);
+    void replay(Recording &r, const double *params, size_t np);
+
     // out = O in (observable applied, Hamiltonians summed); in/out device states
     void apply_obs_to(const Obs &o, const void *in, void *out);
     cd inner(const void *bra, const void *ket);
@@ -275,6 +301,9 @@ class StateVector {
     bool prep_pending_ = false;
     std::vector<cd> pend_;
     void materialize();
+    Recording *capturing_ = nullptr; // execute() records into this recording's buffers
+    std::vector<LOp> lower_recorded(const Recording &r);
+    void capture(Recording &r, Planned pl);
     hipStream_t stream_ = nullptr;
     std::vector<LOp> queue_;
     DevBuf prog_, partials_, slots_, small_;

@@ -56,6 +56,10 @@ _SIGS = {
     "lq_sv_update_data_device": (C.c_int, [_P, _P, C.c_uint64]),
     "lq_sv_get_data_device": (C.c_int, [_P, _P, C.c_uint64]),
     "lq_sv_dlpack": (C.c_int, [_P, C.POINTER(_P)]),
+    "lq_sv_record": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "lq_rec_replay": (C.c_int, [_P, _DP, C.c_uint64]),
+    "lq_rec_stats": (C.c_int, [_P, _U64P, _U64P]),
+    "lq_rec_destroy": (C.c_int, [_P]),
     "lq_dlpack_release": (C.c_int, [_P]),
     "lq_sv_apply_op": (C.c_int, [_P, C.c_char_p, _U32P, C.c_uint32, _U32P, _U8P, C.c_uint32, C.c_int, _DP,
                                  C.c_uint32]),

@@ -264,6 +264,12 @@ class _StateVectorBase:
         check(lib.lq_dlpack_release(p))
         return out
 
+    # -- recorded circuits (HIP graphs) --------------------------------------
+    def record(self, operations: "OpsStruct") -> "Recording":
+        """Prepare |0...0>, apply ``operations`` and keep the program as a HIP
+        graph; ``Recording.replay(params)`` re-runs it with new parameters."""
+        return Recording(self, operations)
+
     def to_torch(self):
         """A torch tensor on the GPU viewing this state's amplitudes (no copy)."""
         import torch
@@ -797,6 +803,36 @@ def _create_ops_list(ops_name, ops_params, ops_wires, ops_inverses, ops_matrices
                      ops_controlled_wires=None, ops_controlled_values=None):
     return OpsStruct(ops_name, ops_params, ops_wires, ops_inverses, ops_matrices,
                      ops_controlled_wires, ops_controlled_values)
+
+
+class Recording:
+    """A circuit captured as a HIP graph on one state (lq_sv_record). ``replay``
+    takes every op's parameters in op order (as many as the recorded ops
+    carry, matrix ops' 0.0 placeholders included) and leaves
+    U(params)|0...0> in the state."""
+
+    def __init__(self, sv: "_StateVectorBase", operations: "OpsStruct"):
+        h = C.c_void_p()
+        check(lib.lq_sv_record(sv._h, operations._h, C.byref(h)))
+        self._h = h
+        self._sv = sv
+        self.nparams = sum(len(p) for p in operations.params)
+
+    def replay(self, params):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
+        check(lib.lq_rec_replay(self._h, p.ctypes.data_as(C.POINTER(C.c_double)) if p.size else None,
+                                C.c_uint64(p.size)))
+
+    def stats(self) -> dict:
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib.lq_rec_stats(self._h, C.byref(a), C.byref(b)))
+        return {"captures": a.value, "replays": b.value}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.lq_rec_destroy(h)
+            self._h = None
 
 
 class AdjointJacobian:
