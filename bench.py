@@ -374,12 +374,13 @@ def run_serving(args):
         kw["small_max_qubits"] = args.serve_small_max
     if args.serve_workers is not None:
         kw["large_workers"] = args.serve_workers
-    sv = serving_sweep(args.requests, qps_batched, qps_plain, server_kw=kw)
+    sv = serving_sweep(args.requests, qps_batched, qps_plain, bits=args.serve_bits, server_kw=kw)
     out = {
         "metric": "adjoint-grad req/s at equal p99, batched vs plain GPU path",
         "value": sv["ratio_at_equal_p99"], "unit": "x plain throughput (both keeping up with arrivals, batched p99 <= plain p99)",
         "n_gpus": 1, "steps": len(sv["sweep"]), "warmup": 64, "ms_per_step": None, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex128" if args.serve_bits == 128 else "complex64",
+        "data": "synthetic",
         "config": {"workload": f"{args.requests} requests, n 10..28 with P(n) ~ 2^-((n-10)/2), depth 4..16 HEA "
                                f"(RZ RY RZ + CZ), 50-term Pauli Hamiltonian, energy + full adjoint gradient",
                    "requests": args.requests, "jit": os.environ.get("LQ_JIT", "1")},
@@ -606,6 +607,8 @@ def main():
     ap.add_argument("--serving-qps-batched", default="300,450,550")
     ap.add_argument("--serving-qps-plain", default="120,170,220")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
+    ap.add_argument("--serve-bits", type=int, choices=[64, 128], default=128,
+                    help="serving workload: state precision (complex64 keeps FP64 reductions)")
     ap.add_argument("--sweep-batched", default="",
                     help="serving workload: arrival rates of the batched server ('none': skip that mode)")
     ap.add_argument("--sweep-plain", default="", help="serving workload: arrival rates of the plain path ('none': skip)")

@@ -276,7 +276,7 @@ typedef struct {
     int small_max_qubits;    /* -1: sizes whose states fit the packed kernel's LDS (12 qubits complex128, 13 complex64); at most 16 */
     int max_batch;           /* requests per packed launch (0: 4096) */
     int batch_streams;       /* packed launches in flight (0: 2) */
-    int large_workers;       /* concurrent engine requests (0: 4) */
+    int large_workers;       /* concurrent engine requests (0: 8) */
     int prep_threads;        /* host threads building packed programs (0: 4) */
 } lq_server_config;
 int lq_server_create(const lq_server_config *cfg, lq_server_t *out);
@@ -339,6 +339,9 @@ int lq_set_mfma_dense5(int on, int *previous);
  * compiled and its estimated state traffic is below one sweep per
  * observable; 2: share whenever compiled; default 1, LQ_ADJOINT_GROUP). */
 int lq_set_engine_option(const char *name, int value, int *previous);
+/* Adjoint backward sweeps run in this process since the last reset: shared
+ * by a group of observables (*grouped) and for one observable (*single). */
+int lq_adjoint_sweep_counts(int reset, uint64_t *grouped, uint64_t *single);
 
 #ifdef __cplusplus
 }

@@ -128,7 +128,11 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BReq *__restrict__ 
                 } else {
                     idx = (((i >> op.t0) & 1u) << 1) | ((i >> op.t1) & 1u);
                 }
-                const V f = idx == 0 ? d[0] : idx == 1 ? d[1] : idx == 2 ? d[2] : d[3];
+                // select by index bits (a select chain on idx compiles to a
+                // dynamically indexed private array in scratch memory)
+                const V f01 = (idx & 1u) ? d[1] : d[0];
+                const V f23 = (idx & 1u) ? d[3] : d[2];
+                const V f = (idx & 2u) ? f23 : f01;
                 if (dot) {
                     acc_dot(ar, ai, S1[i], cmul(f, S0[i]));
                     continue;

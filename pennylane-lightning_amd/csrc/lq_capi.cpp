@@ -776,6 +776,10 @@ int lq_set_engine_option(const char *name, int value, int *previous) {
     });
 }
 
+int lq_adjoint_sweep_counts(int reset, uint64_t *grouped, uint64_t *single) {
+    return guard([&] { adjoint_sweep_counts(reset != 0, grouped, single); });
+}
+
 int lq_probs_noalloc_init(void) {
     return guard([&] { probs_noalloc_init(); });
 }

@@ -1836,6 +1836,13 @@ std::vector<double> backward_sweep(int n, const OpsList &ops, const std::vector<
 }
 
 
+std::atomic<uint64_t> g_sweeps_grouped{0}, g_sweeps_single{0};
+
+void adjoint_sweep_counts(bool reset, uint64_t *grouped, uint64_t *single) {
+    if (grouped) *grouped = reset ? g_sweeps_grouped.exchange(0) : g_sweeps_grouped.load();
+    if (single) *single = reset ? g_sweeps_single.exchange(0) : g_sweeps_single.load();
+}
+
 void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std::shared_ptr<Obs>> &obs,
                       const std::vector<uint64_t> &tp_in, bool apply_operations, double *jac, double *expvals) {
     std::vector<uint64_t> tp = tp_in;
@@ -1907,6 +1914,7 @@ void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std
             states.push_back(h);
         }
         const auto v = sv.execute(w.planned, states, w.nslots);
+        (m > 1 ? g_sweeps_grouped : g_sweeps_single).fetch_add(1, std::memory_order_relaxed);
         for (size_t j = 0; j < m; ++j) {
             if (expvals) expvals[o0 + j] = v[j].real();
             for (size_t t = 0; t < ntp; ++t)

@@ -357,6 +357,9 @@ LOp lower_record(int n, const OpRec &op, bool dagger);
 std::vector<double> backward_sweep(int n, const OpsList &ops, const std::vector<uint64_t> &tp, size_t m, int slot0,
                                    std::vector<LOp> &prog);
 
+// Backward sweeps run since the last reset (process-wide): shared by a
+// group of observables, and for one observable.
+void adjoint_sweep_counts(bool reset, uint64_t *grouped, uint64_t *single);
 void adjoint_jacobian(StateVector &sv, const OpsList &ops, const std::vector<std::shared_ptr<Obs>> &obs,
                       const std::vector<uint64_t> &tp, bool apply_operations, double *jac, double *expvals);
 // w_t = <dy | d psi / d theta_t> for the trainable parameters tp

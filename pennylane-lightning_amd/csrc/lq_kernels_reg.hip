@@ -292,7 +292,11 @@ __device__ __forceinline__ void reg_diag(Regs<T, NS, RB> &R, const TileOp &op, c
             for (int j = 0; j < k; ++j) idx |= static_cast<uint32_t>((g >> op.gpos[j]) & 1ull) << (k - 1 - j);
             if (kind == TOP_PCPHASE) idx = idx < op.pc_dim ? 0u : 1u;
         }
-        const V f = idx == 0 ? d[0] : idx == 1 ? d[1] : idx == 2 ? d[2] : d[3];
+        // two-level select on the index bits (a select chain on idx became a
+        // dynamically indexed private array, i.e. scratch memory)
+        const V f01 = (idx & 1u) ? d[1] : d[0];
+        const V f23 = (idx & 1u) ? d[3] : d[2];
+        const V f = (idx & 2u) ? f23 : f01;
         if (dot) {
             if (ok) acc_dot(accr, acci, pick(R, op.sb, e), cmul(f, pick(R, op.sa, e)));
             continue;

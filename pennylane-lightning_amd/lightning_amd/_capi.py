@@ -57,6 +57,7 @@ _SIGS = {
     "lq_sv_get_data_device": (C.c_int, [_P, _P, C.c_uint64]),
     "lq_sv_dlpack": (C.c_int, [_P, C.POINTER(_P)]),
     "lq_sv_record": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "lq_adjoint_sweep_counts": (C.c_int, [C.c_int, _U64P, _U64P]),
     "lq_rec_replay": (C.c_int, [_P, _DP, C.c_uint64]),
     "lq_rec_stats": (C.c_int, [_P, _U64P, _U64P]),
     "lq_rec_destroy": (C.c_int, [_P]),

@@ -141,6 +141,9 @@ def observable(spec, bits: int = 128):
     if tag == "M":
         return getattr(L, "HermitianObs" + sfx)(np.asarray(spec[1]), [int(w) for w in spec[2]])
     if tag == "H":
+        if len(spec[2]) == 1 and float(spec[1][0]) == 1.0:
+            # a single unit-coefficient term is the term itself (_serialize.py:372-373)
+            return observable(spec[2][0], bits)
         return getattr(L, "Hamiltonian" + sfx)([float(c) for c in spec[1]], [observable(t, bits) for t in spec[2]])
     raise ValueError(f"bad observable {spec!r}")
 

@@ -85,10 +85,18 @@ def set_mfma_dense5(on: bool) -> bool:
 
 def set_engine_option(name: str, value: int) -> int:
     """Set a process-wide engine option ("split_fallback", "mfma_dense5",
-    "product_fold", "plan_search"); returns the previous value."""
+    "product_fold", "plan_search", "adjoint_group"); returns the previous value."""
     prev = C.c_int()
     check(lib.lq_set_engine_option(name.encode(), int(value), C.byref(prev)))
     return prev.value
+
+
+def adjoint_sweep_counts(reset: bool = True) -> dict:
+    """Adjoint backward sweeps run in this process: shared by a group of
+    observables ("grouped") and for one observable ("single")."""
+    g, s = C.c_uint64(), C.c_uint64()
+    check(lib.lq_adjoint_sweep_counts(int(bool(reset)), C.byref(g), C.byref(s)))
+    return {"grouped": g.value, "single": s.value}
 
 
 def jit_stats() -> dict:

@@ -164,4 +164,14 @@ def test_adjoint_observables_share_a_sweep(bits):
         np.testing.assert_allclose(r[0], jexp, atol=tol)
         np.testing.assert_allclose(r[1], eexp, atol=tol)
 
-    run_twice(make, check)
+    # groups whenever their kernels are compiled (the traffic estimate may
+    # prefer one sweep per observable at this size)
+    prev = L.set_engine_option("adjoint_group", 2)
+    try:
+        L.adjoint_sweep_counts(reset=True)
+        run_twice(make, check)
+        counts = L.adjoint_sweep_counts(reset=True)
+    finally:
+        L.set_engine_option("adjoint_group", prev)
+    # first run: four sweeps of one observable; second: one group of three and one single
+    assert counts["grouped"] >= 1 and counts["single"] >= 5, counts

@@ -165,3 +165,11 @@ def test_processing_shapes(A):
     assert len(P(np.array([[1.0, 2.0]]))) == 2
     two = P(np.arange(6.0).reshape(2, 3))
     assert len(two) == 2 and all(len(r) == 3 for r in two)
+
+
+def test_single_unit_term_hamiltonian_is_the_term(A):
+    from lightning_amd import lightning_qubit_ops as L
+    o = A.observable(("H", [1.0], [("N", "PauliZ", 2)]))
+    assert isinstance(o, L.NamedObsC128)
+    h = A.observable(("H", [0.5], [("N", "PauliZ", 2)]))
+    assert isinstance(h, L.HamiltonianC128)
