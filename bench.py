@@ -326,9 +326,14 @@ def serving_sweep(count, qps_batched, qps_plain, bits=128, seed=0, server_kw=Non
         srv = S.Server(bits=bits, batched=batched, **(server_kw or {}))
         srv.replay(warm, np.zeros(len(warm)))
         for q in sweep:
+            st0 = srv.stats()
             res = srv.replay(reqs, S.poisson_arrivals(len(reqs), q, seed=7))
+            st1 = srv.stats()
             summ = S.summarize(res, len(reqs))
-            summ.update({"mode": "batched" if batched else "plain", "offered_qps": q})
+            summ.update({"mode": "batched" if batched else "plain", "offered_qps": q,
+                         "states_made": st1["states_made"] - st0["states_made"],
+                         "states_freed": st1["states_freed"] - st0["states_freed"],
+                         "hbm_in_use_GiB": round(st1["hbm_in_use"] / 2 ** 30, 1)})
             rows.append(summ)
             print(json.dumps({"serving_point": summ}), file=sys.stderr, flush=True)
         srv.close()
@@ -604,7 +609,7 @@ def main():
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--no-serving", action="store_true", help="default workload: skip the serving measurement")
     ap.add_argument("--serving-requests", type=int, default=1024)
-    ap.add_argument("--serving-qps-batched", default="300,450,550")
+    ap.add_argument("--serving-qps-batched", default="450,550,650")
     ap.add_argument("--serving-qps-plain", default="120,170,220")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
     ap.add_argument("--serve-bits", type=int, choices=[64, 128], default=128,

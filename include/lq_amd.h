@@ -297,6 +297,9 @@ int lq_server_replay(lq_server_t srv, uint32_t count, const uint32_t *n, const l
                      double *grads, double *latency_ms, int *path, double *wall_s);
 /* submitted, completed, packed, engine, packed launches, HBM bytes held */
 int lq_server_stats(lq_server_t srv, uint64_t *out6);
+/* Engine states the server allocated and evicted for room since it started
+ * (the workers share one pool of idle states, reused by qubit count). */
+int lq_server_pool_stats(lq_server_t srv, uint64_t *made, uint64_t *freed);
 
 /* ---- Kernel specialisation (no reference counterpart) ----------------------
  * Register tile passes of states with at least LQ_JIT_MIN_QUBITS qubits

@@ -725,6 +725,14 @@ int lq_server_stats(lq_server_t srv, uint64_t *out6) {
     });
 }
 
+int lq_server_pool_stats(lq_server_t srv, uint64_t *made, uint64_t *freed) {
+    return guard([&] {
+        const auto s = srv->srv->stats();
+        if (made) *made = s.states_made;
+        if (freed) *freed = s.states_freed;
+    });
+}
+
 int lq_jit_wait(void) {
     return guard([&] { jit_wait(); });
 }

@@ -220,50 +220,60 @@ void Server::run_engine(StateVector &sv, Req &r) {
 
 void Server::large_loop() {
     (void)hipSetDevice(device_);
-    // states kept per qubit count (with the adjoint's work states), counted
-    // against the HBM budget while held
-    std::map<int, std::unique_ptr<StateVector>> cache;
-    uint64_t held = 0;
+    // Engine states come from a pool shared by the workers: a request takes
+    // an idle state of its size if there is one, else allocates one once the
+    // budget has room, evicting idle states (oldest first) to make it; the
+    // state goes back to the pool afterwards. One pool instead of a cache per
+    // worker: states of a size seen by several workers are not duplicated,
+    // and memory is freed (hipFree waits for the device) only when needed.
     while (true) {
         ReqP r;
+        std::unique_ptr<StateVector> sv;
+        std::vector<std::unique_ptr<StateVector>> evicted;
         {
             std::unique_lock<std::mutex> lk(mu_);
             while (true) {
-                if (stop_) {
-                    hbm_in_use_ -= held;
-                    return;
-                }
+                if (stop_) return;
                 if (!large_ready_.empty()) {
                     const ReqP &h = large_ready_.front();
-                    const uint64_t need = cache.count(h->n) ? 0 : h->bytes;
-                    if (hbm_in_use_ + need <= cfg_.hbm_budget) break;
-                    if (!cache.empty()) {
-                        // make room: drop this worker's other states
-                        lk.unlock();
-                        for (auto &kv : cache) kv.second.reset();
-                        cache.clear();
-                        lk.lock();
-                        hbm_in_use_ -= held;
-                        held = 0;
-                        continue;
+                    auto it = std::find_if(idle_.rbegin(), idle_.rend(), [&](const auto &e) { return e.first == h->n; });
+                    if (it != idle_.rend()) {
+                        sv = std::move(it->second);
+                        idle_.erase(std::next(it).base());
+                        break;
+                    }
+                    while (hbm_in_use_ + h->bytes > cfg_.hbm_budget && !idle_.empty()) {
+                        hbm_in_use_ -= request_bytes(idle_.front().first, cfg_.dtype);
+                        evicted.push_back(std::move(idle_.front().second));
+                        idle_.erase(idle_.begin());
+                        ++stats_.states_freed;
+                    }
+                    if (hbm_in_use_ + h->bytes <= cfg_.hbm_budget) {
+                        hbm_in_use_ += h->bytes;
+                        ++stats_.states_made;
+                        break;
                     }
                 }
                 cv_work_.wait(lk);
             }
             r = large_ready_.front();
             large_ready_.pop_front();
-            if (!cache.count(r->n)) {
-                hbm_in_use_ += r->bytes;
-                held += r->bytes;
-            }
         }
+        evicted.clear(); // frees device memory outside the lock
         try {
-            auto &slot = cache[r->n];
-            if (!slot) slot = std::make_unique<StateVector>(r->n, cfg_.dtype);
-            run_engine(*slot, *r);
+            if (!sv) sv = std::make_unique<StateVector>(r->n, cfg_.dtype);
+            run_engine(*sv, *r);
         } catch (const std::exception &ex) {
             r->res.error = ex.what();
             r->res.path = 1;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (sv) {
+                idle_.emplace_back(r->n, std::move(sv));
+            } else {
+                hbm_in_use_ -= r->bytes; // the allocation failed
+            }
         }
         complete(r);
         cv_work_.notify_all();

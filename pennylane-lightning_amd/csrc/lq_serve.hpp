@@ -10,7 +10,8 @@
 //     that became ready while a launch ran joins the next one;
 //   * larger ones run through the engine (fused tile passes + adjoint sweep)
 //     on `large_workers` threads with their own streams, admitted while the
-//     HBM they need (3 states of 2^n amplitudes) fits the budget.
+//     HBM they need (3 states of 2^n amplitudes) fits the budget; their
+//     engine states come from a pool of idle states the workers share.
 // In plain mode one worker runs every request to completion, in arrival
 // order, through the engine: the one-request-at-a-time baseline.
 #pragma once
@@ -67,6 +68,7 @@ class Server {
     struct Stats {
         uint64_t submitted = 0, completed = 0, packed = 0, engine = 0, launches = 0;
         uint64_t hbm_in_use = 0;
+        uint64_t states_made = 0, states_freed = 0; // engine states allocated / evicted for room
     };
     Stats stats();
 
@@ -93,6 +95,9 @@ class Server {
     std::unordered_map<uint64_t, ReqP> live_;
     uint64_t next_id_ = 1;
     uint64_t hbm_in_use_ = 0;
+    // engine states not in use, shared by the workers (oldest first), each
+    // counted against the HBM budget until it is freed
+    std::vector<std::pair<int, std::unique_ptr<StateVector>>> idle_;
     Stats stats_;
     bool stop_ = false;
     std::vector<std::thread> threads_;

@@ -52,6 +52,8 @@ lib.lq_server_replay.argtypes = [_P, C.c_uint32, _U32P, C.POINTER(_P), C.POINTER
 lib.lq_server_replay.restype = C.c_int
 lib.lq_server_stats.argtypes = [_P, _U64P]
 lib.lq_server_stats.restype = C.c_int
+lib.lq_server_pool_stats.argtypes = [_P, _U64P, _U64P]
+lib.lq_server_pool_stats.restype = C.c_int
 
 
 @dataclass
@@ -183,7 +185,11 @@ class Server:
         out = np.zeros(6, dtype=np.uint64)
         check(lib.lq_server_stats(self._h, out.ctypes.data_as(_U64P)))
         keys = ("submitted", "completed", "packed", "engine", "launches", "hbm_in_use")
-        return {k: int(v) for k, v in zip(keys, out)}
+        st = {k: int(v) for k, v in zip(keys, out)}
+        made, freed = C.c_uint64(), C.c_uint64()
+        check(lib.lq_server_pool_stats(self._h, C.byref(made), C.byref(freed)))
+        st.update({"states_made": made.value, "states_freed": freed.value})
+        return st
 
 
 def summarize(res: dict, count: int) -> dict:
