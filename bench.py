@@ -39,6 +39,10 @@ import time
 
 import numpy as np
 
+# before anything starts the HIP runtime: one hardware queue per stream for
+# the serving workloads (the engine binding sets the same default)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, "pennylane-lightning_amd")):
     if _p not in sys.path:
@@ -318,13 +322,24 @@ def serving_sweep(count, qps_batched, qps_plain, bits=128, seed=0, server_kw=Non
         r.handles()
     setup_s = time.perf_counter() - t0
     warm = [r for r in reqs if r.n <= 14][:64]
+    warm_big = [r for r in S.serving_requests(count, seed=seed + 1, bits=bits) if r.n > 12]
+    for r in warm_big:
+        r.handles()
     rows = []
     # the batched sweep runs first: specialised kernels compiled in the
     # background during a sweep stay cached, so any such gain goes to the
     # plain path
     for batched, sweep in ((True, qps_batched), (False, qps_plain)):
+        if not sweep:
+            continue
         srv = S.Server(bits=bits, batched=batched, **(server_kw or {}))
         srv.replay(warm, np.zeros(len(warm)))
+        # untimed warm-up of the server itself: the engine-path requests of
+        # another stream of the same distribution (other seed), so the pool
+        # holds states of the sizes that occur and the structures that recur
+        # are planned before the first timed point (without it the first
+        # point of a sweep ran cold, up to 2x slower)
+        srv.replay(warm_big, np.zeros(len(warm_big)))
         for q in sweep:
             st0 = srv.stats()
             res = srv.replay(reqs, S.poisson_arrivals(len(reqs), q, seed=7))
@@ -379,6 +394,10 @@ def run_serving(args):
         kw["small_max_qubits"] = args.serve_small_max
     if args.serve_workers is not None:
         kw["large_workers"] = args.serve_workers
+    if args.serve_prep is not None:
+        kw["prep_threads"] = args.serve_prep
+    if args.serve_streams is not None:
+        kw["batch_streams"] = args.serve_streams
     sv = serving_sweep(args.requests, qps_batched, qps_plain, bits=args.serve_bits, server_kw=kw)
     out = {
         "metric": "adjoint-grad req/s at equal p99, batched vs plain GPU path",
@@ -609,8 +628,8 @@ def main():
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--no-serving", action="store_true", help="default workload: skip the serving measurement")
     ap.add_argument("--serving-requests", type=int, default=1024)
-    ap.add_argument("--serving-qps-batched", default="450,550,650")
-    ap.add_argument("--serving-qps-plain", default="120,170,220")
+    ap.add_argument("--serving-qps-batched", default="550,750,950")
+    ap.add_argument("--serving-qps-plain", default="170,220,260")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
     ap.add_argument("--serve-bits", type=int, choices=[64, 128], default=128,
                     help="serving workload: state precision (complex64 keeps FP64 reductions)")
@@ -619,6 +638,8 @@ def main():
     ap.add_argument("--sweep-plain", default="", help="serving workload: arrival rates of the plain path ('none': skip)")
     ap.add_argument("--serve-small-max", type=int, default=None,
                     help="serving: largest qubit count run in the packed kernel (default: the server's)")
+    ap.add_argument("--serve-prep", type=int, default=None, help="serving workload: packed-program host threads")
+    ap.add_argument("--serve-streams", type=int, default=None, help="serving workload: packed launches in flight")
     ap.add_argument("--serve-workers", type=int, default=None,
                     help="serving: concurrent engine workers for larger requests (default: the server's)")
     args = ap.parse_args()

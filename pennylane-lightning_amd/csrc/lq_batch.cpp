@@ -131,6 +131,14 @@ bool build_batch_program(int n, int dtype, const OpsList &ops, const Obs &ham, c
 BatchRunner::BatchRunner() {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate");
+    // sized up front for launches of a few hundred requests (a request's
+    // program is ~30-40 KiB): growing a buffer frees the old one, and
+    // hipFree / hipHostFree wait for the whole device, i.e. for every engine
+    // request in flight. Under a backlog the launches grow, and regrowing
+    // then stalled the packed path for as long as the engine work took.
+    hin_.ensure(16u << 20);
+    dbuf_.ensure(16u << 20);
+    hout_.ensure(1u << 20);
 }
 
 BatchRunner::~BatchRunner() {

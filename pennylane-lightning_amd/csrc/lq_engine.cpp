@@ -35,11 +35,21 @@ void DevBuf::release() {
     p = nullptr;
     bytes = 0;
 }
+// Buffers grow by at least half their size (rounded to 64 KiB) so that a
+// run of slightly larger programs reallocates them a few times, not every
+// time: hipFree and hipHostFree wait for the whole device, which stalls the
+// other streams (concurrent requests of a server) on every reallocation.
+static size_t grown_size(size_t have, size_t need) {
+    const size_t g = std::max(need, have + have / 2);
+    return (g + 65535) & ~static_cast<size_t>(65535);
+}
+
 void DevBuf::ensure(size_t b) {
     if (b <= bytes) return;
+    const size_t nb = grown_size(bytes, b);
     release();
-    hip_check(hipMalloc(&p, b), "hipMalloc");
-    bytes = b;
+    hip_check(hipMalloc(&p, nb), "hipMalloc");
+    bytes = nb;
 }
 
 HostPinned::~HostPinned() {
@@ -47,10 +57,12 @@ HostPinned::~HostPinned() {
 }
 void HostPinned::ensure(size_t b) {
     if (b <= bytes) return;
+    const size_t nb = grown_size(bytes, b);
     if (p) (void)hipHostFree(p);
     p = nullptr;
-    hip_check(hipHostMalloc(&p, b, hipHostMallocDefault), "hipHostMalloc");
-    bytes = b;
+    bytes = 0;
+    hip_check(hipHostMalloc(&p, nb, hipHostMallocDefault), "hipHostMalloc");
+    bytes = nb;
 }
 
 EngineTuning &tuning() {
@@ -226,6 +238,7 @@ bool obs_to_pauli(const Obs &o, int n, std::vector<PauliWord> *out) {
 StateVector::StateVector(int n, int dtype) : n_(n), dtype_(dtype ? 1 : 0) {
     LQ_ABORT_IF(n < 1 || n > 40, "number of qubits must be in [1, 40]");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    own_stream_ = stream_;
     hip_check(hipEventCreateWithFlags(&staged_, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventCreateWithFlags(&copied_, hipEventDisableTiming), "hipEventCreate");
     hipError_t e = hipMalloc(&d_, bytes());
@@ -234,6 +247,7 @@ StateVector::StateVector(int n, int dtype) : n_(n), dtype_(dtype ? 1 : 0) {
         (void)hipEventDestroy(copied_);
         (void)hipEventDestroy(staged_);
         (void)hipStreamDestroy(stream_);
+        own_stream_ = stream_ = nullptr;
         hip_check(e, "hipMalloc(state vector)");
     }
     reset();
@@ -250,7 +264,16 @@ StateVector::~StateVector() {
     if (d_) (void)hipFree(d_);
     if (staged_) (void)hipEventDestroy(staged_);
     if (copied_) (void)hipEventDestroy(copied_);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (own_stream_) {
+        (void)hipStreamSynchronize(own_stream_);
+        (void)hipStreamDestroy(own_stream_);
+    }
+}
+
+void StateVector::bind_stream(hipStream_t s) {
+    flush();
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    stream_ = s ? s : own_stream_;
 }
 
 hipEvent_t StateVector::take_event() {

@@ -156,6 +156,11 @@ class StateVector {
         return d_;
     }
     hipStream_t stream() const { return stream_; }
+    // Run this state's work on `s` from now on (nullptr: its own stream),
+    // after waiting for the work queued so far. A server binds the states
+    // it takes from its pool to the worker's stream, so its streams stay as
+    // few as its workers (each one a hardware queue of its own).
+    void bind_stream(hipStream_t s);
 
     // state preparation (lq/StateVectorLQubit.hpp:928-1036)
     void reset();
@@ -305,6 +310,7 @@ class StateVector {
     std::vector<LOp> lower_recorded(const Recording &r);
     void capture(Recording &r, Planned pl);
     hipStream_t stream_ = nullptr;
+    hipStream_t own_stream_ = nullptr; // created with the state; stream_ may be a caller's
     std::vector<LOp> queue_;
     DevBuf prog_, partials_, slots_, small_;
     HostPinned staging_;

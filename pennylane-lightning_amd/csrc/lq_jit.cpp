@@ -2,6 +2,9 @@
 #include "lq_jit.hpp"
 
 #include <hip/hiprtc.h>
+#include <sys/resource.h>
+#include <sys/syscall.h>
+#include <unistd.h>
 
 #include <atomic>
 #include <condition_variable>
@@ -1077,6 +1080,7 @@ JitConfig &jit_config() {
         if (const char *s = std::getenv("LQ_JIT_SYNC")) j.sync = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_MIN_QUBITS")) j.min_qubits = std::atoi(s);
         if (const char *s = std::getenv("LQ_JIT_THREADS")) j.threads = std::max(1, std::atoi(s));
+        if (const char *s = std::getenv("LQ_JIT_NICE")) j.nice = std::atoi(s);
         if (const char *s = std::getenv("LQ_JIT_NT")) j.nontemporal = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_SPLIT")) j.split_rotations = std::atoi(s) != 0;
         if (const char *s = std::getenv("LQ_JIT_LDS32")) j.lds32 = std::atoi(s) != 0;
@@ -1253,6 +1257,11 @@ class Service {
     }
 
     void worker() {
+        // compiles are background work: they yield the CPU to the threads
+        // that run circuits and requests (an unprivileged thread may lower
+        // its own priority; a refusal leaves it as it was)
+        if (jit_config().nice > 0)
+            (void)setpriority(PRIO_PROCESS, static_cast<id_t>(syscall(SYS_gettid)), jit_config().nice);
         while (true) {
             std::string src;
             std::shared_ptr<JEntry> e;

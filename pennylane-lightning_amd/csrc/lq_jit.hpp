@@ -29,6 +29,7 @@ struct JitConfig {
     bool sync = false;      // compile before first use instead of in the background
     int min_qubits = 16;    // smaller states always use the generic kernels
     int threads = 8;        // background compile threads
+    int nice = 10;          // their scheduling niceness (LQ_JIT_NICE; 0: unchanged)
     bool nontemporal = true;  // state loads/stores marked nontemporal (streamed, not retained)
     bool split_rotations = true; // 2x2 gates as phase / real rotation / phase in specialised passes
     bool scale_rotations = true; // specialised passes: rotations as [[1, -t], [t, 1]]

@@ -220,6 +220,18 @@ void Server::run_engine(StateVector &sv, Req &r) {
 
 void Server::large_loop() {
     (void)hipSetDevice(device_);
+    // the worker's own stream: the pooled states it takes run on it, so the
+    // server's streams are its workers and packed runners only, each with a
+    // hardware queue of its own (states keep their own streams while idle)
+    hipStream_t ws = nullptr;
+    hip_check(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking), "hipStreamCreate");
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    } guard{ws};
     // Engine states come from a pool shared by the workers: a request takes
     // an idle state of its size if there is one, else allocates one once the
     // budget has room, evicting idle states (oldest first) to make it; the
@@ -262,18 +274,24 @@ void Server::large_loop() {
         evicted.clear(); // frees device memory outside the lock
         try {
             if (!sv) sv = std::make_unique<StateVector>(r->n, cfg_.dtype);
+            sv->bind_stream(ws);
             run_engine(*sv, *r);
         } catch (const std::exception &ex) {
             r->res.error = ex.what();
             r->res.path = 1;
         }
+        if (sv) {
+            try {
+                sv->bind_stream(nullptr); // idle states hold their own stream
+            } catch (const std::exception &ex) {
+                if (r->res.error.empty()) r->res.error = ex.what();
+                sv.reset(); // a state whose stream failed is not reused
+            }
+        }
         {
             std::lock_guard<std::mutex> lk(mu_);
-            if (sv) {
-                idle_.emplace_back(r->n, std::move(sv));
-            } else {
-                hbm_in_use_ -= r->bytes; // the allocation failed
-            }
+            if (sv) idle_.emplace_back(r->n, std::move(sv));
+            else hbm_in_use_ -= r->bytes; // not allocated, or dropped: no longer held
         }
         complete(r);
         cv_work_.notify_all();

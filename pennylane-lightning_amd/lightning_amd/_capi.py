@@ -18,6 +18,16 @@ if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built: run `make -C pennylane-lightning_amd` "
                       "(or __graft_entry__.build()) first")
 
+# Hardware queues per process: HIP maps streams onto GPU_MAX_HW_QUEUES
+# hardware queues (default 4), and each queue runs its kernels in order, so
+# a stream sharing a queue waits behind every kernel another stream put
+# there. A server runs its engine workers and packed launches on about ten
+# streams; with 4 queues a packed launch waited behind a large request's
+# passes and the batched server's throughput collapsed at random (p50 of
+# packed requests 100-1000 ms); with 16 each stream has its own queue.
+# Set before the HIP runtime starts (the caller's value is kept).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 # torch (when installed) is imported first: the engine's HIP runtime
 # libraries have the same sonames as the ones torch bundles, so the loader
 # binds the engine to torch's copies and the process has one HIP runtime.

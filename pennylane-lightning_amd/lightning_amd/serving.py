@@ -203,6 +203,12 @@ def summarize(res: dict, count: int) -> dict:
     out = {"requests": count, "wall_s": res["wall_s"], "req_per_s": count / res["wall_s"] if res["wall_s"] else None,
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "mean_ms": float(lat.mean()), "packed": int((np.asarray(res["path"]) == 0).sum())}
+    paths = np.asarray(res["path"])
+    for code, name in ((0, "packed"), (1, "engine")):
+        sel = lat[paths == code]
+        if sel.size:
+            out[f"{name}_p50_ms"] = float(np.percentile(sel, 50))
+            out[f"{name}_p99_ms"] = float(np.percentile(sel, 99))
     arr = res.get("arrivals_s")
     if arr is not None and len(arr) > 1:
         arr = np.asarray(arr, dtype=np.float64)
