@@ -628,7 +628,7 @@ def main():
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--no-serving", action="store_true", help="default workload: skip the serving measurement")
     ap.add_argument("--serving-requests", type=int, default=1024)
-    ap.add_argument("--serving-qps-batched", default="550,750,950")
+    ap.add_argument("--serving-qps-batched", default="550,750,950,1150")
     ap.add_argument("--serving-qps-plain", default="170,220,260")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
     ap.add_argument("--serve-bits", type=int, choices=[64, 128], default=128,
