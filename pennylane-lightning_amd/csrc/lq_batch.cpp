@@ -131,14 +131,6 @@ bool build_batch_program(int n, int dtype, const OpsList &ops, const Obs &ham, c
 BatchRunner::BatchRunner() {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate");
-    // sized up front for launches of a few hundred requests (a request's
-    // program is ~30-40 KiB): growing a buffer frees the old one, and
-    // hipFree / hipHostFree wait for the whole device, i.e. for every engine
-    // request in flight. Under a backlog the launches grow, and regrowing
-    // then stalled the packed path for as long as the engine work took.
-    hin_.ensure(16u << 20);
-    dbuf_.ensure(16u << 20);
-    hout_.ensure(1u << 20);
 }
 
 BatchRunner::~BatchRunner() {
@@ -181,10 +173,15 @@ void BatchRunner::launch(const std::vector<const BatchProgram *> &progs, int dty
     const size_t out_at = align(terms_at + nterms * sizeof(PauliTerm));
     const size_t total_in = out_at;
     const size_t total = out_at + nslots * sizeof(double2);
-    // the previous launch on this runner has finished (finish() was called)
-    hin_.ensure(total_in);
-    hout_.ensure(nslots * sizeof(double2));
-    dbuf_.ensure(total);
+    // the previous launch on this runner has finished (finish() was called).
+    // The buffers start at sizes for launches of a few hundred requests (a
+    // request's program is ~30-40 KiB): growing a buffer frees the old one,
+    // and hipFree / hipHostFree wait for the whole device, i.e. for every
+    // engine request in flight; under a backlog the launches grow, and
+    // regrowing then stalled the packed path for as long as that work took.
+    hin_.ensure(std::max<size_t>(total_in, 16u << 20));
+    hout_.ensure(std::max<size_t>(nslots * sizeof(double2), 1u << 20));
+    dbuf_.ensure(std::max<size_t>(total, 16u << 20));
     if (nscratch) scratch_.ensure(nscratch * (dtype ? 16 : 8));
     char *h = static_cast<char *>(hin_.p);
     BReq *rq = reinterpret_cast<BReq *>(h + reqs_at);

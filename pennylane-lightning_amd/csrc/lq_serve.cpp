@@ -223,11 +223,14 @@ void Server::large_loop() {
     // the worker's own stream: the pooled states it takes run on it, so the
     // server's streams are its workers and packed runners only, each with a
     // hardware queue of its own (states keep their own streams while idle)
+    // (if it cannot be created, requests run on the states' own streams:
+    // nothing may throw out of a worker thread)
     hipStream_t ws = nullptr;
-    hip_check(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking), "hipStreamCreate");
+    if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) ws = nullptr;
     struct StreamGuard {
         hipStream_t s;
         ~StreamGuard() {
+            if (!s) return;
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
         }
