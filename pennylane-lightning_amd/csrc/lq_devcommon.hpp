@@ -38,6 +38,13 @@ template <typename V> __device__ __forceinline__ V cmul2(V a, V b, V c, V d) {
     r.y = a.x * b.y + a.y * b.x + c.x * d.y + c.y * d.x;
     return r;
 }
+// a*b + c*d for real a, c (the real parts of a and c are used)
+template <typename V> __device__ __forceinline__ V rmul2(V a, V b, V c, V d) {
+    V r;
+    r.x = a.x * b.x + c.x * d.x;
+    r.y = a.x * b.y + c.x * d.y;
+    return r;
+}
 template <typename V> __device__ __forceinline__ V cneg(V a) {
     V r;
     r.x = -a.x;

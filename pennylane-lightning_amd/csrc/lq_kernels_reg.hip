@@ -129,7 +129,25 @@ __device__ __forceinline__ void reg_dense1(Regs<T, NS, RB> &R, const TileOp &op,
         });
         return;
     }
+    // real matrices (RY, X of a CNOT, H, ...) take half the arithmetic; the
+    // test is on the op's coefficients, uniform over the workgroup
+    const bool real = m00.y == T(0) && m01.y == T(0) && m10.y == T(0) && m11.y == T(0);
     if (gcm == 0) {
+        if (real) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (!((op.smask >> s) & 1u)) continue;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if (e & (1 << I)) continue;
+                    const int e1 = e | (1 << I);
+                    const V a0 = R.v[s][e], a1 = R.v[s][e1];
+                    R.v[s][e] = rmul2(m00, a0, m01, a1);
+                    R.v[s][e1] = rmul2(m10, a0, m11, a1);
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             if (!((op.smask >> s) & 1u)) continue;
@@ -140,6 +158,23 @@ __device__ __forceinline__ void reg_dense1(Regs<T, NS, RB> &R, const TileOp &op,
                 const V a0 = R.v[s][e], a1 = R.v[s][e1];
                 R.v[s][e] = cmul2(m00, a0, m01, a1);
                 R.v[s][e1] = cmul2(m10, a0, m11, a1);
+            }
+        }
+        return;
+    }
+    if (real) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (e & (1 << I)) continue;
+            const int e1 = e | (1 << I);
+            const bool ok = (gidx<RB>(gbt, rg, e) & gcm) == gcv;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (!((op.smask >> s) & 1u)) continue;
+                const V a0 = R.v[s][e], a1 = R.v[s][e1];
+                const V n0 = rmul2(m00, a0, m01, a1), n1 = rmul2(m10, a0, m11, a1);
+                R.v[s][e] = ok ? n0 : a0;
+                R.v[s][e1] = ok ? n1 : a1;
             }
         }
         return;
