@@ -41,7 +41,7 @@ def emu_binary(tmp_path_factory):
     (0, 1, 1, 2, 16), (1, 0, 1, 2, 16), (2, 1, 1, 2, 16), (0, 1, 1, 3, 14), (1, 0, 1, 3, 14),
     (3, 0, 1, 1, 17), (3, 0, 1, 3, 17), (3, 1, 1, 3, 17), (2, 1, 1, 4, 14), (1, 0, 1, 4, 14), (0, 1, 1, 5, 14),
     (0, 1, 1, 6, 14), (2, 1, 1, 6, 14), (1, 0, 1, 6, 14), (1, 0, 1, 7, 14), (3, 0, 1, 7, 17),
-    (4, 1, 1, 1, 14), (4, 0, 1, 1, 14), (5, 1, 1, 1, 15), (5, 0, 1, 1, 14)])
+    (4, 1, 1, 1, 13), (4, 0, 1, 1, 13), (5, 1, 1, 1, 13), (5, 0, 1, 1, 13)])
 def test_generated_kernels_match_naive_application(emu_binary, case, dtype, lds32, look, n):
     exe, work = emu_binary
     # lds32: transposes through 32 KiB of LDS (emulated at exactly the size the kernel declares);
