@@ -20,7 +20,8 @@ throughput whose p99 is within the plain path's p99 to the plain path's
 best throughput that keeps up with its arrivals. A point keeps up when the
 requests completed while requests still arrive come to >= 90% of the
 offered rate (steady_req_per_s); req_per_s (all requests over first arrival
-to last completion) is reported beside it.
+to last completion) is reported beside it. serving_c64: the same sweep with
+complex64 states (FP64 reductions).
 
 cpu_baseline: the reference lightning.qubit C++ (oracle/_ref/liblq_ref.so,
 built from the reference sources) on the host cores, on the first gates of
@@ -628,6 +629,7 @@ def main():
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--no-serving", action="store_true", help="default workload: skip the serving measurement")
     ap.add_argument("--serving-requests", type=int, default=1024)
+    ap.add_argument("--no-serving-c64", action="store_true", help="skip the complex64 serving sweep")
     ap.add_argument("--serving-qps-batched", default="550,750,950,1150")
     ap.add_argument("--serving-qps-plain", default="170,220,260")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
@@ -736,11 +738,16 @@ def main():
     copy_bps = copy_rate(sv)
     cpu = None if args.no_cpu else cpu_baseline(n, ops, args.cpu_seconds, elem)
     traffic = MEASURED_TRAFFIC.get((args.precision, n))
-    serving = None
+    serving = serving_c64 = None
     if not args.no_serving:
         del sv, M
         serving = serving_sweep(args.serving_requests, [float(x) for x in args.serving_qps_batched.split(",")],
                                 [float(x) for x in args.serving_qps_plain.split(",")])
+        if not args.no_serving_c64:
+            # the same stream with complex64 states (FP64 reductions)
+            serving_c64 = serving_sweep(args.serving_requests,
+                                        [float(x) for x in args.serving_qps_batched.split(",")],
+                                        [float(x) for x in args.serving_qps_plain.split(",")], bits=64)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -773,6 +780,7 @@ def main():
         "expvals": {"Z0Z1": ev[0], f"X{n-1}": ev[1]},
         "cpu_baseline": cpu,
         "serving": serving,
+        "serving_c64": serving_c64,
     }
     print(json.dumps(out))
 
