@@ -399,6 +399,8 @@ def run_serving(args):
         kw["prep_threads"] = args.serve_prep
     if args.serve_streams is not None:
         kw["batch_streams"] = args.serve_streams
+    if args.serve_lane is not None:
+        kw["lane_workers"] = args.serve_lane
     sv = serving_sweep(args.requests, qps_batched, qps_plain, bits=args.serve_bits, server_kw=kw)
     out = {
         "metric": "adjoint-grad req/s at equal p99, batched vs plain GPU path",
@@ -630,7 +632,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="default workload: skip the serving measurement")
     ap.add_argument("--serving-requests", type=int, default=1024)
     ap.add_argument("--no-serving-c64", action="store_true", help="skip the complex64 serving sweep")
-    ap.add_argument("--serving-qps-batched", default="550,750,950,1150")
+    ap.add_argument("--serving-qps-batched", default="550,750,950,1150,1350")
     ap.add_argument("--serving-qps-plain", default="170,220,260")
     ap.add_argument("--qps", default="", help="comma-separated arrival rates for the serving sweep")
     ap.add_argument("--serve-bits", type=int, choices=[64, 128], default=128,
@@ -642,6 +644,8 @@ def main():
                     help="serving: largest qubit count run in the packed kernel (default: the server's)")
     ap.add_argument("--serve-prep", type=int, default=None, help="serving workload: packed-program host threads")
     ap.add_argument("--serve-streams", type=int, default=None, help="serving workload: packed launches in flight")
+    ap.add_argument("--serve-lane", type=int, default=None,
+                    help="serving workload: engine workers reserved for requests of <= 16 qubits (negative: none)")
     ap.add_argument("--serve-workers", type=int, default=None,
                     help="serving: concurrent engine workers for larger requests (default: the server's)")
     args = ap.parse_args()

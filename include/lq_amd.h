@@ -278,6 +278,9 @@ typedef struct {
     int batch_streams;       /* packed launches in flight (0: 2) */
     int large_workers;       /* concurrent engine requests (0: 8) */
     int prep_threads;        /* host threads building packed programs (0: 4) */
+    int lane_workers;        /* engine workers that take only requests of <= lane_max_qubits, oldest
+                                first, besides the large_workers (0: 2; negative: none) */
+    int lane_max_qubits;     /* (0: 16) */
 } lq_server_config;
 int lq_server_create(const lq_server_config *cfg, lq_server_t *out);
 int lq_server_destroy(lq_server_t srv);

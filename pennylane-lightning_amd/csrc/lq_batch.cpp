@@ -128,9 +128,32 @@ bool build_batch_program(int n, int dtype, const OpsList &ops, const Obs &ham, c
     return true;
 }
 
+namespace {
+// Initial sizes of a runner's buffers: 64 MiB of programs is ~2000 requests
+// (a request's program is ~30-40 KiB), more than a server backlog of its
+// default 4096-request cap reaches in practice; results are 16 bytes a slot.
+constexpr size_t kRunnerInBytes = 64u << 20;
+constexpr size_t kRunnerOutBytes = 1u << 20;
+constexpr size_t kSectionAlign = 256; // each of the 5 sections starts 256-byte aligned
+} // namespace
+
 BatchRunner::BatchRunner() {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate");
+    hin_.ensure(kRunnerInBytes);
+    dbuf_.ensure(kRunnerInBytes);
+    hout_.ensure(kRunnerOutBytes);
+}
+
+size_t BatchRunner::program_bytes(const BatchProgram &p) {
+    return sizeof(BReq) + p.ops.size() * sizeof(BOp) + p.coef.size() * sizeof(double2) +
+           p.terms.size() * sizeof(PauliTerm) + p.nslots * sizeof(double2);
+}
+
+size_t BatchRunner::capacity() const {
+    const size_t slack = 5 * kSectionAlign;
+    const size_t in = std::min(hin_.bytes, dbuf_.bytes);
+    return in > slack ? in - slack : 0;
 }
 
 BatchRunner::~BatchRunner() {
@@ -174,14 +197,13 @@ void BatchRunner::launch(const std::vector<const BatchProgram *> &progs, int dty
     const size_t total_in = out_at;
     const size_t total = out_at + nslots * sizeof(double2);
     // the previous launch on this runner has finished (finish() was called).
-    // The buffers start at sizes for launches of a few hundred requests (a
-    // request's program is ~30-40 KiB): growing a buffer frees the old one,
-    // and hipFree / hipHostFree wait for the whole device, i.e. for every
-    // engine request in flight; under a backlog the launches grow, and
-    // regrowing then stalled the packed path for as long as that work took.
-    hin_.ensure(std::max<size_t>(total_in, 16u << 20));
-    hout_.ensure(std::max<size_t>(nslots * sizeof(double2), 1u << 20));
-    dbuf_.ensure(std::max<size_t>(total, 16u << 20));
+    // Growing a buffer frees the old one, and hipFree / hipHostFree wait for
+    // the whole device, i.e. for every engine request in flight: the server
+    // keeps its launches within capacity(), so these only grow for a single
+    // program larger than the initial sizes.
+    hin_.ensure(total_in);
+    hout_.ensure(nslots * sizeof(double2));
+    dbuf_.ensure(total);
     if (nscratch) scratch_.ensure(nscratch * (dtype ? 16 : 8));
     char *h = static_cast<char *>(hin_.p);
     BReq *rq = reinterpret_cast<BReq *>(h + reqs_at);

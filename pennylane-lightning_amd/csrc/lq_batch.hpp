@@ -91,6 +91,12 @@ class BatchRunner {
     // Wait, then energies[i] and grads[i][t] for the programs of the last launch.
     void finish(std::vector<double> *energies, std::vector<std::vector<double>> *grads);
     hipStream_t stream() const { return stream_; }
+    // Bytes one program adds to a launch's device buffer.
+    static size_t program_bytes(const BatchProgram &p);
+    // Program bytes (program_bytes summed) a launch can take without growing
+    // this runner's buffers: growing frees the old ones, and hipFree /
+    // hipHostFree wait for the whole device.
+    size_t capacity() const;
 
   private:
     hipStream_t stream_ = nullptr;

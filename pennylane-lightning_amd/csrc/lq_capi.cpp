@@ -616,6 +616,8 @@ int lq_server_create(const lq_server_config *cfg, lq_server_t *out) {
         if (cfg->batch_streams > 0) c.batch_streams = cfg->batch_streams;
         if (cfg->large_workers > 0) c.large_workers = cfg->large_workers;
         if (cfg->prep_threads > 0) c.prep_threads = cfg->prep_threads;
+        if (cfg->lane_workers != 0) c.lane_workers = std::max(0, cfg->lane_workers);
+        if (cfg->lane_max_qubits > 0) c.lane_max_qubits = cfg->lane_max_qubits;
         auto *h = new lq_server_s;
         h->srv = std::make_unique<lqamd::Server>(c);
         *out = h;

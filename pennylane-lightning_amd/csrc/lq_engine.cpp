@@ -32,13 +32,17 @@ void hip_check(hipError_t e, const char *what) {
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
     if (p) (void)hipFree(p);
+    for (void *o : outgrown_) (void)hipFree(o);
+    outgrown_.clear();
     p = nullptr;
     bytes = 0;
 }
 // Buffers grow by at least half their size (rounded to 64 KiB) so that a
 // run of slightly larger programs reallocates them a few times, not every
-// time: hipFree and hipHostFree wait for the whole device, which stalls the
-// other streams (concurrent requests of a server) on every reallocation.
+// time; the outgrown ones below kKeepOutgrown are kept (at most twice the
+// final size in all), because hipFree and hipHostFree wait for the whole
+// device: a server's worker freeing one waited for every request in flight
+// (up to ~1 s) while the requests queued behind it waited too.
 static size_t grown_size(size_t have, size_t need) {
     const size_t g = std::max(need, have + have / 2);
     return (g + 65535) & ~static_cast<size_t>(65535);
@@ -47,18 +51,23 @@ static size_t grown_size(size_t have, size_t need) {
 void DevBuf::ensure(size_t b) {
     if (b <= bytes) return;
     const size_t nb = grown_size(bytes, b);
-    release();
+    if (p && bytes <= kKeepOutgrown) outgrown_.push_back(p);
+    else if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
     hip_check(hipMalloc(&p, nb), "hipMalloc");
     bytes = nb;
 }
 
 HostPinned::~HostPinned() {
     if (p) (void)hipHostFree(p);
+    for (void *o : outgrown_) (void)hipHostFree(o);
 }
 void HostPinned::ensure(size_t b) {
     if (b <= bytes) return;
     const size_t nb = grown_size(bytes, b);
-    if (p) (void)hipHostFree(p);
+    if (p && bytes <= kKeepOutgrown) outgrown_.push_back(p);
+    else if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
     hip_check(hipHostMalloc(&p, nb, hipHostMallocDefault), "hipHostMalloc");

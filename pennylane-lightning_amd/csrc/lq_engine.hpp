@@ -18,6 +18,12 @@ namespace lqamd {
 
 void hip_check(hipError_t e, const char *what);
 
+// Device and pinned host buffers that only grow. hipFree and hipHostFree
+// wait for the whole device, so a buffer outgrown while other streams run
+// (the concurrent requests of a server) is kept until the buffer itself is
+// freed when it is small (kKeepOutgrown); larger ones are freed at once.
+constexpr size_t kKeepOutgrown = 64u << 20;
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -27,6 +33,8 @@ struct DevBuf {
     ~DevBuf();
     void ensure(size_t b);
     void release();
+  private:
+    std::vector<void *> outgrown_;
 };
 
 struct HostPinned {
@@ -37,6 +45,8 @@ struct HostPinned {
     HostPinned &operator=(const HostPinned &) = delete;
     ~HostPinned();
     void ensure(size_t b);
+  private:
+    std::vector<void *> outgrown_;
 };
 
 // Observables (core/observables/Observables.hpp:128-584)

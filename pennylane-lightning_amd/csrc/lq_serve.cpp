@@ -26,12 +26,14 @@ Server::Server(const ServerConfig &cfg) : cfg_(cfg) {
     cfg_.batch_streams = std::max(1, cfg_.batch_streams);
     cfg_.large_workers = std::max(1, cfg_.large_workers);
     cfg_.prep_threads = std::max(1, cfg_.prep_threads);
+    cfg_.lane_workers = std::max(0, cfg_.lane_workers);
     if (cfg_.batched) {
         for (int i = 0; i < cfg_.prep_threads; ++i) threads_.emplace_back([this] { prep_loop(); });
         threads_.emplace_back([this] { batch_loop(); });
-        for (int i = 0; i < cfg_.large_workers; ++i) threads_.emplace_back([this] { large_loop(); });
+        for (int i = 0; i < cfg_.large_workers; ++i) threads_.emplace_back([this] { large_loop(false); });
+        for (int i = 0; i < cfg_.lane_workers; ++i) threads_.emplace_back([this] { large_loop(true); });
     } else {
-        threads_.emplace_back([this] { large_loop(); });
+        threads_.emplace_back([this] { large_loop(false); });
     }
 }
 
@@ -175,7 +177,18 @@ void Server::batch_loop() {
             for (int i = 0; i < K; ++i) busy = busy || !flight[i].empty();
             if (stop_ && !busy) return;
             if (free_slot >= 0 && !small_ready_.empty() && !stop_) {
-                const size_t cnt = std::min(small_ready_.size(), static_cast<size_t>(cfg_.max_batch));
+                // at most max_batch requests, and no more program bytes than
+                // the runner holds without reallocating (a reallocation waits
+                // for every engine request in flight; one launch of ~2000
+                // requests already fills the device several times over)
+                const size_t cap = runners[free_slot]->capacity();
+                size_t cnt = 0, bytes = 0;
+                while (cnt < small_ready_.size() && cnt < static_cast<size_t>(cfg_.max_batch)) {
+                    const size_t b = BatchRunner::program_bytes(small_ready_[cnt]->prog);
+                    if (cnt > 0 && bytes + b > cap) break;
+                    bytes += b;
+                    ++cnt;
+                }
                 take.assign(small_ready_.begin(), small_ready_.begin() + static_cast<long>(cnt));
                 small_ready_.erase(small_ready_.begin(), small_ready_.begin() + static_cast<long>(cnt));
             } else {
@@ -218,7 +231,7 @@ void Server::run_engine(StateVector &sv, Req &r) {
     r.res.path = 1;
 }
 
-void Server::large_loop() {
+void Server::large_loop(bool lane) {
     (void)hipSetDevice(device_);
     // the worker's own stream: the pooled states it takes run on it, so the
     // server's streams are its workers and packed runners only, each with a
@@ -245,12 +258,18 @@ void Server::large_loop() {
         ReqP r;
         std::unique_ptr<StateVector> sv;
         std::vector<std::unique_ptr<StateVector>> evicted;
+        size_t k = 0; // position of the request taken in large_ready_
         {
             std::unique_lock<std::mutex> lk(mu_);
             while (true) {
                 if (stop_) return;
-                if (!large_ready_.empty()) {
-                    const ReqP &h = large_ready_.front();
+                // a lane worker takes the oldest request of its sizes, the
+                // others the oldest request
+                k = 0;
+                if (lane)
+                    while (k < large_ready_.size() && large_ready_[k]->n > cfg_.lane_max_qubits) ++k;
+                if (k < large_ready_.size()) {
+                    const ReqP &h = large_ready_[k];
                     auto it = std::find_if(idle_.rbegin(), idle_.rend(), [&](const auto &e) { return e.first == h->n; });
                     if (it != idle_.rend()) {
                         sv = std::move(it->second);
@@ -271,8 +290,8 @@ void Server::large_loop() {
                 }
                 cv_work_.wait(lk);
             }
-            r = large_ready_.front();
-            large_ready_.pop_front();
+            r = large_ready_[k];
+            large_ready_.erase(large_ready_.begin() + static_cast<long>(k));
         }
         evicted.clear(); // frees device memory outside the lock
         try {

@@ -11,7 +11,9 @@
 //   * larger ones run through the engine (fused tile passes + adjoint sweep)
 //     on `large_workers` threads with their own streams, admitted while the
 //     HBM they need (3 states of 2^n amplitudes) fits the budget; their
-//     engine states come from a pool of idle states the workers share.
+//     engine states come from a pool of idle states the workers share;
+//     `lane_workers` more threads take only the engine requests of up to
+//     `lane_max_qubits` qubits, which would otherwise wait behind large ones.
 // In plain mode one worker runs every request to completion, in arrival
 // order, through the engine: the one-request-at-a-time baseline.
 #pragma once
@@ -43,6 +45,13 @@ struct ServerConfig {
     // the same rates with 4 and 8 workers and its p99 was lower with 8
     int large_workers = 8;
     int prep_threads = 4;
+    // engine workers that take only requests of <= lane_max_qubits (oldest
+    // first), so that they do not queue behind the large ones: at 1150
+    // requests/s the 13-16-qubit engine requests (~0.1-1 ms of GPU work
+    // each, 28% of the stream) waited 360-410 ms (p50) behind 17-28-qubit
+    // requests in the shared first-come queue
+    int lane_workers = 2;
+    int lane_max_qubits = 16;
 };
 
 struct ServeResult {
@@ -106,7 +115,7 @@ class Server {
     void complete(const ReqP &r);
     void prep_loop();
     void batch_loop();
-    void large_loop();
+    void large_loop(bool lane);
     void run_engine(StateVector &sv, Req &r);
 };
 

@@ -36,7 +36,8 @@ _IP = C.POINTER(C.c_int)
 class _Config(C.Structure):
     _fields_ = [("dtype", C.c_int), ("batched", C.c_int), ("hbm_budget", C.c_uint64),
                 ("small_max_qubits", C.c_int), ("max_batch", C.c_int), ("batch_streams", C.c_int),
-                ("large_workers", C.c_int), ("prep_threads", C.c_int)]
+                ("large_workers", C.c_int), ("prep_threads", C.c_int), ("lane_workers", C.c_int),
+                ("lane_max_qubits", C.c_int)]
 
 
 lib.lq_server_create.argtypes = [C.POINTER(_Config), C.POINTER(_P)]
@@ -117,9 +118,11 @@ def poisson_arrivals(count: int, qps: float, seed: int = 1) -> np.ndarray:
 
 class Server:
     def __init__(self, bits: int = 128, batched: bool = True, hbm_budget: int = 0, small_max_qubits: int = -1,
-                 max_batch: int = 0, batch_streams: int = 0, large_workers: int = 0, prep_threads: int = 0):
+                 max_batch: int = 0, batch_streams: int = 0, large_workers: int = 0, prep_threads: int = 0,
+                 lane_workers: int = 0, lane_max_qubits: int = 0):
         cfg = _Config(1 if bits == 128 else 0, int(bool(batched)), int(hbm_budget), int(small_max_qubits),
-                      int(max_batch), int(batch_streams), int(large_workers), int(prep_threads))
+                      int(max_batch), int(batch_streams), int(large_workers), int(prep_threads),
+                      int(lane_workers), int(lane_max_qubits))
         self._h = C.c_void_p()
         check(lib.lq_server_create(C.byref(cfg), C.byref(self._h)))
         self.bits = bits

@@ -105,6 +105,25 @@ def test_plain_matches_batched():
         np.testing.assert_allclose(a, b, atol=1e-10)
 
 
+@pytest.mark.parametrize("lane", [-1, 3])
+def test_lane_workers_match_plain(lane):
+    """Engine requests of <= 16 qubits run on the lane workers (3 here) while
+    one general worker holds 17- and 18-qubit requests; results equal the plain
+    path's, and with no lane (-1) everything runs on the general worker."""
+    reqs = small_requests(3, seed=11, n_lo=17, n_hi=18) + small_requests(6, seed=12, n_lo=13, n_hi=16)
+    plain = S.Server(bits=128, batched=False)
+    rp = plain.replay(reqs, np.zeros(len(reqs)))
+    plain.close()
+    srv = S.Server(bits=128, batched=True, large_workers=1, lane_workers=lane)
+    rb = srv.replay(reqs, np.linspace(0.0, 0.001, len(reqs)))
+    st = srv.stats()
+    srv.close()
+    assert st["engine"] == len(reqs) and set(rb["path"].tolist()) == {1}
+    np.testing.assert_allclose(rb["energies"], rp["energies"], atol=1e-10)
+    for a, b in zip(rb["grads"], rp["grads"]):
+        np.testing.assert_allclose(a, b, atol=1e-10)
+
+
 def test_serving_workload_sample():
     """A sample of the serving workload (n in 10..16 here) through both paths."""
     reqs = S.serving_requests(24, seed=2, n_max=16, depth_max=6)
