@@ -303,6 +303,13 @@ int lq_server_stats(lq_server_t srv, uint64_t *out6);
 /* Engine states the server allocated and evicted for room since it started
  * (the workers share one pool of idle states, reused by qubit count). */
 int lq_server_pool_stats(lq_server_t srv, uint64_t *made, uint64_t *freed);
+/* Packed-path timing (no reference counterpart: server telemetry), longest
+ * since the server started or the last reset != 0 call, in ms: out6[0] a
+ * request's wait from its program being ready to its launch, out6[1] the host
+ * time of one launch call, out6[2] one launch's time until the batcher saw it
+ * done, out6[3..5] that launch's upload, kernel and download times on the
+ * GPU (event timestamps). */
+int lq_server_timing(lq_server_t srv, double *out6, int reset);
 
 /* ---- Kernel specialisation (no reference counterpart) ----------------------
  * Register tile passes of states with at least LQ_JIT_MIN_QUBITS qubits

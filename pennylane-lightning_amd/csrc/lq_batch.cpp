@@ -3,6 +3,7 @@
 #include "lq_batch.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 namespace lqamd {
@@ -138,8 +139,31 @@ constexpr size_t kSectionAlign = 256; // each of the 5 sections starts 256-byte 
 } // namespace
 
 BatchRunner::BatchRunner() {
-    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    // a hardware queue of its own: a CU-masked stream (every CU enabled) gets
+    // a new queue instead of sharing one with other streams once the
+    // process's queue pool is full, where it would wait behind their kernels
+    int dev = 0, cus = 0;
+    const char *oq = std::getenv("LQ_BATCH_OWN_QUEUE");
+    const char *ku = std::getenv("LQ_BATCH_KERNEL_UPLOAD");
+    kernel_upload_ = !ku || std::atoi(ku) != 0;
+    if ((!oq || std::atoi(oq) != 0) && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) {
+        std::vector<uint32_t> mask((static_cast<size_t>(cus) + 31) / 32, 0u);
+        for (int c = 0; c < cus; ++c) mask[static_cast<size_t>(c) / 32] |= 1u << (c % 32);
+        if (hipExtStreamCreateWithCUMask(&stream_, static_cast<uint32_t>(mask.size()), mask.data()) != hipSuccess)
+            stream_ = nullptr;
+    }
+    if (!stream_) hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate");
+    for (hipEvent_t *e : {&t0_, &t_in_, &t_k_, &t_out_}) hip_check(hipEventCreate(e), "hipEventCreate");
+    // programs and results move through kernels that read or write host
+    // memory on this runner's queue: a copy-engine transfer waited in the
+    // copy engine's queue behind other streams' transfers, themselves
+    // waiting for those streams' kernels (uploads of ~1 MB and downloads of
+    // a few KB took up to 1.3 s under load). Fine-grained (coherent) host
+    // memory: the GPU does not cache it, so a launch reads what the host just
+    // wrote and the host reads what the kernel wrote once the launch is done.
+    if (kernel_upload_) hin_.flags = hout_.flags = hipHostMallocCoherent;
     hin_.ensure(kRunnerInBytes);
     dbuf_.ensure(kRunnerInBytes);
     hout_.ensure(kRunnerOutBytes);
@@ -162,6 +186,12 @@ BatchRunner::~BatchRunner() {
         (void)hipStreamDestroy(stream_);
     }
     if (done_) (void)hipEventDestroy(done_);
+    for (hipEvent_t e : {t0_, t_in_, t_k_, t_out_})
+        if (e) (void)hipEventDestroy(e);
+}
+
+void BatchRunner::segments(double out3[3]) const {
+    for (int i = 0; i < 3; ++i) out3[i] = seg_[i];
 }
 
 void BatchRunner::launch(const std::vector<const BatchProgram *> &progs, int dtype) {
@@ -241,7 +271,16 @@ void BatchRunner::launch(const std::vector<const BatchProgram *> &progs, int dty
     }
     out_slots_ = nslots;
     char *d = static_cast<char *>(dbuf_.p);
-    hip_check(hipMemcpyAsync(d, h, total_in, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(batch)");
+    hip_check(hipEventRecord(t0_, stream_), "hipEventRecord");
+    void *hin_dev = nullptr;
+    if (kernel_upload_ && hipHostGetDevicePointer(&hin_dev, hin_.p, 0) != hipSuccess) hin_dev = nullptr;
+    if (hin_dev) {
+        launch_copy16(hin_dev, d, total_in, stream_);
+        hip_check(hipGetLastError(), "batch upload launch");
+    } else {
+        hip_check(hipMemcpyAsync(d, h, total_in, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(batch)");
+    }
+    hip_check(hipEventRecord(t_in_, stream_), "hipEventRecord");
     const BReq *dr = reinterpret_cast<const BReq *>(d + reqs_at);
     const BOp *dop = reinterpret_cast<const BOp *>(d + ops_at);
     const double2 *dcf = reinterpret_cast<const double2 *>(d + coef_at);
@@ -253,8 +292,17 @@ void BatchRunner::launch(const std::vector<const BatchProgram *> &progs, int dty
         launch_batch(dr + n_lds, dop, dcf, dtm, dout, scratch_.p, static_cast<uint32_t>(nreq - n_lds), max_n, false,
                      dtype, stream_);
     hip_check(hipGetLastError(), "batch launch");
-    hip_check(hipMemcpyAsync(hout_.p, d + out_at, nslots * sizeof(double2), hipMemcpyDeviceToHost, stream_),
-              "hipMemcpyAsync(batch results)");
+    hip_check(hipEventRecord(t_k_, stream_), "hipEventRecord");
+    void *hout_dev = nullptr;
+    if (hin_dev && hipHostGetDevicePointer(&hout_dev, hout_.p, 0) != hipSuccess) hout_dev = nullptr;
+    if (hout_dev) {
+        launch_copy16(d + out_at, hout_dev, nslots * sizeof(double2), stream_);
+        hip_check(hipGetLastError(), "batch download launch");
+    } else {
+        hip_check(hipMemcpyAsync(hout_.p, d + out_at, nslots * sizeof(double2), hipMemcpyDeviceToHost, stream_),
+                  "hipMemcpyAsync(batch results)");
+    }
+    hip_check(hipEventRecord(t_out_, stream_), "hipEventRecord");
     hip_check(hipEventRecord(done_, stream_), "hipEventRecord");
 }
 
@@ -272,6 +320,10 @@ void BatchRunner::finish(std::vector<double> *energies, std::vector<std::vector<
     grads->clear();
     if (progs_.empty()) return;
     hip_check(hipEventSynchronize(done_), "hipEventSynchronize");
+    float ms[3] = {0.f, 0.f, 0.f};
+    if (hipEventElapsedTime(&ms[0], t0_, t_in_) == hipSuccess && hipEventElapsedTime(&ms[1], t_in_, t_k_) == hipSuccess &&
+        hipEventElapsedTime(&ms[2], t_k_, t_out_) == hipSuccess)
+        for (int i = 0; i < 3; ++i) seg_[i] = ms[i];
     const double2 *r = static_cast<const double2 *>(hout_.p);
     for (size_t i = 0; i < progs_.size(); ++i) {
         const BatchProgram &p = *progs_[i];

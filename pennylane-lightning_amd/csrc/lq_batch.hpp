@@ -56,6 +56,10 @@ inline int batch_max_qubits(int) { return 16; }
 
 // in_lds: every request has n <= batch_lds_qubits (max_n sizes the LDS);
 // otherwise the states live in `scratch` at each request's state_off.
+// Copy `bytes` (a multiple of 16) with a kernel on stream `s` (no copy engine
+// involved); either side may be device-visible host memory.
+void launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t s);
+
 void launch_batch(const BReq *reqs, const BOp *ops, const double2 *coef, const PauliTerm *terms, double2 *out,
                   void *scratch, uint32_t nreq, int max_n, bool in_lds, int dtype, hipStream_t s);
 
@@ -91,6 +95,9 @@ class BatchRunner {
     // Wait, then energies[i] and grads[i][t] for the programs of the last launch.
     void finish(std::vector<double> *energies, std::vector<std::vector<double>> *grads);
     hipStream_t stream() const { return stream_; }
+    // GPU time of the last finished launch's upload, kernels and download, ms
+    // (device timestamps of events on the runner's stream)
+    void segments(double out3[3]) const;
     // Bytes one program adds to a launch's device buffer.
     static size_t program_bytes(const BatchProgram &p);
     // Program bytes (program_bytes summed) a launch can take without growing
@@ -101,6 +108,9 @@ class BatchRunner {
   private:
     hipStream_t stream_ = nullptr;
     hipEvent_t done_ = nullptr;
+    hipEvent_t t0_ = nullptr, t_in_ = nullptr, t_k_ = nullptr, t_out_ = nullptr;
+    bool kernel_upload_ = true;
+    double seg_[3] = {0.0, 0.0, 0.0};
     DevBuf dbuf_, scratch_;
     HostPinned hin_, hout_;
     std::vector<const BatchProgram *> progs_;

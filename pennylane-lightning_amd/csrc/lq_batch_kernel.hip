@@ -7,6 +7,8 @@
 // across the four waves, into the request's result slots.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lq_batch.hpp"
 #include "lq_devcommon.hpp"
 
@@ -183,6 +185,23 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BReq *__restrict__ 
 }
 
 } // namespace
+
+namespace {
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                     size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256u + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * 256u)
+        dst[i] = src[i];
+}
+} // namespace
+
+void launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 16;
+    if (n == 0) return;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 1024));
+    hipLaunchKernelGGL(copy16_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const uint4 *>(src),
+                       static_cast<uint4 *>(dst), n);
+}
 
 void launch_batch(const BReq *reqs, const BOp *ops, const double2 *coef, const PauliTerm *terms, double2 *out,
                   void *scratch, uint32_t nreq, int max_n, bool in_lds, int dtype, hipStream_t s) {

@@ -735,6 +735,18 @@ int lq_server_pool_stats(lq_server_t srv, uint64_t *made, uint64_t *freed) {
     });
 }
 
+int lq_server_timing(lq_server_t srv, double *out6, int reset) {
+    return guard([&] {
+        LQ_ABORT_IF(!srv || !out6, "null argument");
+        const auto s = srv->srv->stats();
+        out6[0] = s.ready_wait_max_ms;
+        out6[1] = s.launch_call_max_ms;
+        out6[2] = s.launch_max_ms;
+        for (int j = 0; j < 3; ++j) out6[3 + j] = s.slow_launch_seg_ms[j];
+        if (reset) srv->srv->reset_timing();
+    });
+}
+
 int lq_jit_wait(void) {
     return guard([&] { jit_wait(); });
 }

@@ -70,7 +70,7 @@ void HostPinned::ensure(size_t b) {
     else if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
-    hip_check(hipHostMalloc(&p, nb, hipHostMallocDefault), "hipHostMalloc");
+    hip_check(hipHostMalloc(&p, nb, flags), "hipHostMalloc");
     bytes = nb;
 }
 

@@ -40,6 +40,7 @@ struct DevBuf {
 struct HostPinned {
     void *p = nullptr;
     size_t bytes = 0;
+    unsigned flags = hipHostMallocDefault; // hipHostMalloc flags of the next allocation
     HostPinned() = default;
     HostPinned(const HostPinned &) = delete;
     HostPinned &operator=(const HostPinned &) = delete;

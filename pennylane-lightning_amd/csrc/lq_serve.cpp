@@ -91,6 +91,12 @@ Server::Stats Server::stats() {
     return s;
 }
 
+void Server::reset_timing() {
+    std::lock_guard<std::mutex> lk(mu_);
+    stats_.ready_wait_max_ms = stats_.launch_call_max_ms = stats_.launch_max_ms = 0.0;
+    for (double &v : stats_.slow_launch_seg_ms) v = 0.0;
+}
+
 void Server::complete(const ReqP &r) {
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -127,6 +133,7 @@ void Server::prep_loop() {
         }
         {
             std::lock_guard<std::mutex> lk(mu_);
+            r->t_ready = Clock::now();
             (small ? small_ready_ : large_ready_).push_back(r);
         }
         cv_work_.notify_all();
@@ -139,6 +146,10 @@ void Server::batch_loop() {
     std::vector<std::unique_ptr<BatchRunner>> runners;
     for (int i = 0; i < K; ++i) runners.emplace_back(std::make_unique<BatchRunner>());
     std::vector<std::vector<ReqP>> flight(K);
+    std::vector<Clock::time_point> launched(K);
+    auto ms_since = [](Clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+    };
     auto retire = [&](int i, bool block) {
         if (flight[i].empty()) return false;
         std::vector<double> e;
@@ -150,6 +161,16 @@ void Server::batch_loop() {
             runners[i]->finish(&e, &g);
         } catch (const std::exception &ex) {
             err = ex.what();
+        }
+        {
+            const double t = ms_since(launched[i]);
+            double seg[3];
+            runners[i]->segments(seg);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (t > stats_.launch_max_ms) {
+                stats_.launch_max_ms = t;
+                for (int j = 0; j < 3; ++j) stats_.slow_launch_seg_ms[j] = seg[j];
+            }
         }
         for (size_t j = 0; j < flight[i].size(); ++j) {
             Req &r = *flight[i][j];
@@ -191,6 +212,7 @@ void Server::batch_loop() {
                 }
                 take.assign(small_ready_.begin(), small_ready_.begin() + static_cast<long>(cnt));
                 small_ready_.erase(small_ready_.begin(), small_ready_.begin() + static_cast<long>(cnt));
+                for (const auto &r : take) stats_.ready_wait_max_ms = std::max(stats_.ready_wait_max_ms, ms_since(r->t_ready));
             } else {
                 // poll the launches in flight; wake early for new work
                 cv_work_.wait_for(lk, std::chrono::microseconds(busy ? 20 : 2000), [&] {
@@ -203,9 +225,12 @@ void Server::batch_loop() {
         progs.reserve(take.size());
         for (const auto &r : take) progs.push_back(&r->prog);
         try {
+            launched[free_slot] = Clock::now();
             runners[free_slot]->launch(progs, cfg_.dtype);
             flight[free_slot] = std::move(take);
+            const double t = ms_since(launched[free_slot]);
             std::lock_guard<std::mutex> lk(mu_);
+            stats_.launch_call_max_ms = std::max(stats_.launch_call_max_ms, t);
             ++stats_.launches;
         } catch (const std::exception &ex) {
             for (const auto &r : take) {

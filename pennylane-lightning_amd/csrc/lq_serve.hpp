@@ -78,8 +78,16 @@ class Server {
         uint64_t submitted = 0, completed = 0, packed = 0, engine = 0, launches = 0;
         uint64_t hbm_in_use = 0;
         uint64_t states_made = 0, states_freed = 0; // engine states allocated / evicted for room
+        // packed path, longest since the last reset: a request's wait between
+        // its program being ready and its launch, the host time inside one
+        // launch call, and one launch's time from that call to the batcher
+        // seeing it done
+        double ready_wait_max_ms = 0.0, launch_call_max_ms = 0.0, launch_max_ms = 0.0;
+        // the GPU segments (upload, kernels, download) of that longest launch
+        double slow_launch_seg_ms[3] = {0.0, 0.0, 0.0};
     };
     Stats stats();
+    void reset_timing();
 
   private:
     using Clock = std::chrono::steady_clock;
@@ -89,7 +97,7 @@ class Server {
         OpsList ops;
         std::shared_ptr<Obs> ham;
         std::vector<uint64_t> tp;
-        Clock::time_point t0;
+        Clock::time_point t0, t_ready;
         BatchProgram prog;
         uint64_t bytes = 0;
         ServeResult res;

@@ -55,6 +55,8 @@ lib.lq_server_stats.argtypes = [_P, _U64P]
 lib.lq_server_stats.restype = C.c_int
 lib.lq_server_pool_stats.argtypes = [_P, _U64P, _U64P]
 lib.lq_server_pool_stats.restype = C.c_int
+lib.lq_server_timing.argtypes = [_P, _DP, C.c_int]
+lib.lq_server_timing.restype = C.c_int
 
 
 @dataclass
@@ -183,6 +185,17 @@ class Server:
         grads = [gr[int(off[i]):int(off[i + 1])] for i in range(count)]
         return {"energies": en, "grads": grads, "latency_ms": lat, "path": path, "wall_s": wall.value,
                 "arrivals_s": arr.copy()}
+
+    def timing(self, reset: bool = True) -> dict:
+        """Packed-path timing maxima (ms) since the start or the last reset:
+        a request's wait from ready to launch, the host time of a launch call,
+        a launch's time until the batcher saw it done, and that launch's
+        upload / kernel / download time on the GPU."""
+        out = np.zeros(6)
+        check(lib.lq_server_timing(self._h, out.ctypes.data_as(_DP), int(bool(reset))))
+        return {"ready_wait_max_ms": float(out[0]), "launch_call_max_ms": float(out[1]),
+                "launch_max_ms": float(out[2]), "slow_launch_upload_ms": float(out[3]),
+                "slow_launch_kernel_ms": float(out[4]), "slow_launch_download_ms": float(out[5])}
 
     def stats(self) -> dict:
         out = np.zeros(6, dtype=np.uint64)

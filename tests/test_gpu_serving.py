@@ -124,6 +124,31 @@ def test_lane_workers_match_plain(lane):
         np.testing.assert_allclose(a, b, atol=1e-10)
 
 
+@pytest.mark.parametrize("kernel_copies", ["1", "0"])
+def test_packed_transfers_and_timing(kernel_copies, monkeypatch):
+    """Packed programs and results move through copy kernels on the runner's
+    own queue (default) or through hipMemcpyAsync (LQ_BATCH_KERNEL_UPLOAD=0,
+    LQ_BATCH_OWN_QUEUE=0, read when the server starts): same results against
+    the oracle either way; the timing counters are consistent and reset."""
+    monkeypatch.setenv("LQ_BATCH_KERNEL_UPLOAD", kernel_copies)
+    monkeypatch.setenv("LQ_BATCH_OWN_QUEUE", kernel_copies)
+    reqs = small_requests(10, seed=13, n_lo=3, n_hi=12)
+    srv = S.Server(bits=128, batched=True)
+    res = srv.replay(reqs, np.linspace(0.0, 0.003, len(reqs)))
+    tm = srv.timing(reset=True)
+    again = srv.timing(reset=False)
+    srv.close()
+    assert set(res["path"].tolist()) == {0}
+    for i, r in enumerate(reqs):
+        e, g = oracle_result(r)
+        assert abs(res["energies"][i] - e) < 1e-10
+        np.testing.assert_allclose(res["grads"][i], g, atol=1e-10)
+    assert tm["launch_max_ms"] > 0.0 and tm["ready_wait_max_ms"] >= 0.0 and tm["launch_call_max_ms"] > 0.0
+    seg = tm["slow_launch_upload_ms"] + tm["slow_launch_kernel_ms"] + tm["slow_launch_download_ms"]
+    assert 0.0 < seg <= tm["launch_max_ms"] + 1e-3
+    assert all(v == 0.0 for v in again.values())
+
+
 def test_serving_workload_sample():
     """A sample of the serving workload (n in 10..16 here) through both paths."""
     reqs = S.serving_requests(24, seed=2, n_max=16, depth_max=6)
