@@ -1,5 +1,6 @@
 // Device state vector and executor (host side).
 #include "lq_engine.hpp"
+#include "lq_batch.hpp"
 #include "lq_jit.hpp"
 
 #include <algorithm>
@@ -244,8 +245,31 @@ bool obs_to_pauli(const Obs &o, int n, std::vector<PauliWord> *out) {
 }
 
 // ---------------------------------------------------------------------------
+namespace {
+// Program uploads and dot-result readbacks of execute() go through copy
+// kernels on the state's stream and fine-grained host memory instead of the
+// copy engines (LQ_KERNEL_COPIES=0: hipMemcpyAsync): the copy engines'
+// queues are shared by every stream's transfers, so with concurrent states
+// (a request server's workers) a transfer waited behind other streams'
+// transfers, which wait for those streams' kernels.
+bool kernel_copies() {
+    static const bool v = [] {
+        const char *e = std::getenv("LQ_KERNEL_COPIES");
+        return !e || std::atoi(e) != 0;
+    }();
+    return v;
+}
+
+// Device address of fine-grained host memory, or nullptr.
+void *host_dev_ptr(void *h) {
+    void *d = nullptr;
+    return hipHostGetDevicePointer(&d, h, 0) == hipSuccess ? d : nullptr;
+}
+} // namespace
+
 StateVector::StateVector(int n, int dtype) : n_(n), dtype_(dtype ? 1 : 0) {
     LQ_ABORT_IF(n < 1 || n > 40, "number of qubits must be in [1, 40]");
+    if (kernel_copies()) staging_.flags = readback_.flags = hipHostMallocCoherent;
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     own_stream_ = stream_;
     hip_check(hipEventCreateWithFlags(&staged_, hipEventDisableTiming), "hipEventCreate");
@@ -837,7 +861,15 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
     if (form_basis) prep_pending_ = false;
     if (!ranges.empty()) std::memcpy(hs + ranges_at, ranges.data(), ranges.size() * sizeof(SlotRange));
     pbuf.ensure(total);
-    hip_check(hipMemcpyAsync(pbuf.p, hs, total, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
+    // both buffers hold a multiple of 64 KiB (grown_size), so the copy may
+    // round up to 16 bytes
+    void *hs_dev = !capturing_ && kernel_copies() ? host_dev_ptr(hs) : nullptr;
+    if (hs_dev) {
+        launch_copy16(hs_dev, pbuf.p, (total + 15) & ~static_cast<size_t>(15), stream_);
+        hip_check(hipGetLastError(), "program upload launch");
+    } else {
+        hip_check(hipMemcpyAsync(pbuf.p, hs, total, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
+    }
     if (!capturing_) hip_check(hipEventRecord(staged_, stream_), "hipEventRecord");
     if (part_n) partials_.ensure(part_n * sizeof(double2));
     if (capturing_) {
@@ -916,10 +948,22 @@ std::vector<cd> StateVector::execute(const Planned &pl, const std::vector<void *
         launch_reduce_slots(static_cast<const double2 *>(partials_.p),
                             reinterpret_cast<const SlotRange *>(dp + ranges_at), static_cast<uint32_t>(nslots),
                             static_cast<double2 *>(slots_.p), stream_);
+        const size_t rb = static_cast<size_t>(nslots) * sizeof(double2);
+        void *rb_dev = nullptr;
+        if (kernel_copies()) {
+            readback_.ensure(rb);
+            rb_dev = host_dev_ptr(readback_.p);
+        }
         std::vector<double2> h(static_cast<size_t>(nslots));
-        hip_check(hipMemcpyAsync(h.data(), slots_.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_),
-                  "hipMemcpyAsync(slots)");
-        hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        if (rb_dev) {
+            launch_copy16(slots_.p, rb_dev, rb, stream_);
+            hip_check(hipGetLastError(), "slot readback launch");
+            hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            std::memcpy(h.data(), readback_.p, rb);
+        } else {
+            hip_check(hipMemcpyAsync(h.data(), slots_.p, rb, hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync(slots)");
+            hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        }
         for (int i = 0; i < nslots; ++i) result[i] = cd(h[i].x, h[i].y);
     }
     return result;

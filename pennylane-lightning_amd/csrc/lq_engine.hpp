@@ -325,6 +325,7 @@ class StateVector {
     std::vector<LOp> queue_;
     DevBuf prog_, partials_, slots_, small_;
     HostPinned staging_;
+    HostPinned readback_; // dot results, written by a copy kernel (kernel_copies)
     hipEvent_t staged_ = nullptr;
     hipEvent_t copied_ = nullptr; // a copy out of another state finished reading it
     std::vector<std::unique_ptr<DevBuf>> ws_;
