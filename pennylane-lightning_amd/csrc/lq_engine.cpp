@@ -269,14 +269,16 @@ void *host_dev_ptr(void *h) {
 
 StateVector::StateVector(int n, int dtype) : n_(n), dtype_(dtype ? 1 : 0) {
     LQ_ABORT_IF(n < 1 || n > 40, "number of qubits must be in [1, 40]");
-    if (kernel_copies()) staging_.flags = readback_.flags = hipHostMallocCoherent;
+    if (kernel_copies()) staging_.flags = readback_.flags = bounce_.flags = hipHostMallocCoherent;
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     own_stream_ = stream_;
     hip_check(hipEventCreateWithFlags(&staged_, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventCreateWithFlags(&copied_, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&bounced_, hipEventDisableTiming), "hipEventCreate");
     hipError_t e = hipMalloc(&d_, bytes());
     if (e != hipSuccess) {
         d_ = nullptr;
+        (void)hipEventDestroy(bounced_);
         (void)hipEventDestroy(copied_);
         (void)hipEventDestroy(staged_);
         (void)hipStreamDestroy(stream_);
@@ -284,6 +286,43 @@ StateVector::StateVector(int n, int dtype) : n_(n), dtype_(dtype ? 1 : 0) {
         hip_check(e, "hipMalloc(state vector)");
     }
     reset();
+}
+
+void StateVector::upload_small(void *dev, size_t dev_cap, const void *host, size_t bytes) {
+    if (bytes == 0) return;
+    const size_t r = (bytes + 15) & ~static_cast<size_t>(15);
+    void *bd = nullptr;
+    if (kernel_copies() && r <= dev_cap) {
+        hip_check(hipEventSynchronize(bounced_), "hipEventSynchronize"); // bounce_ is free again
+        bounce_.ensure(r);
+        bd = host_dev_ptr(bounce_.p);
+    }
+    if (bd) {
+        std::memcpy(bounce_.p, host, bytes);
+        launch_copy16(bd, dev, r, stream_);
+        hip_check(hipGetLastError(), "upload launch");
+        hip_check(hipEventRecord(bounced_, stream_), "hipEventRecord");
+    } else {
+        hip_check(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync");
+    }
+}
+
+void StateVector::read_back(void *host, const void *dev, size_t bytes) {
+    if (bytes == 0) return;
+    void *rd = nullptr;
+    if (kernel_copies() && bytes % 16 == 0) {
+        readback_.ensure(bytes);
+        rd = host_dev_ptr(readback_.p);
+    }
+    if (rd) {
+        launch_copy16(dev, rd, bytes, stream_);
+        hip_check(hipGetLastError(), "readback launch");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        std::memcpy(host, readback_.p, bytes);
+    } else {
+        hip_check(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream_), "memcpy");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+    }
 }
 
 StateVector::~StateVector() {
@@ -297,6 +336,7 @@ StateVector::~StateVector() {
     if (d_) (void)hipFree(d_);
     if (staged_) (void)hipEventDestroy(staged_);
     if (copied_) (void)hipEventDestroy(copied_);
+    if (bounced_) (void)hipEventDestroy(bounced_);
     if (own_stream_) {
         (void)hipStreamSynchronize(own_stream_);
         (void)hipStreamDestroy(own_stream_);
@@ -1266,8 +1306,7 @@ double StateVector::norm2() {
         launch_norm2<decltype(t)>(data(), n_, static_cast<double2 *>(partials_.p), nb, stream_);
     });
     std::vector<double2> h(nb);
-    hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
-    hip_check(hipStreamSynchronize(stream_), "sync");
+    read_back(h.data(), partials_.p, nb * sizeof(double2));
     double s = 0.0, c = 0.0;
     for (const auto &v : h) {
         const double t = s + v.x;
@@ -1287,8 +1326,7 @@ cd StateVector::overlap(StateVector &other) {
     partials_.ensure(nb * sizeof(double2));
     launch_overlap(data(), dtype_, op, other.dtype_, n_, static_cast<double2 *>(partials_.p), nb, stream_);
     std::vector<double2> h(nb);
-    hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
-    hip_check(hipStreamSynchronize(stream_), "sync");
+    read_back(h.data(), partials_.p, nb * sizeof(double2));
     double sr = 0.0, cr = 0.0, si = 0.0, ci = 0.0;
     for (const auto &v : h) {
         double t = sr + v.x;
@@ -1321,8 +1359,7 @@ std::vector<double> StateVector::reduce_rows_to_host(uint32_t rows, uint32_t col
     slots_.ensure(static_cast<size_t>(cols) * sizeof(double));
     launch_reduce_rows(static_cast<const double *>(partials_.p), rows, cols, static_cast<double *>(slots_.p), stream_);
     std::vector<double> h(cols);
-    hip_check(hipMemcpyAsync(h.data(), slots_.p, cols * sizeof(double), hipMemcpyDeviceToHost, stream_), "memcpy");
-    hip_check(hipStreamSynchronize(stream_), "sync");
+    read_back(h.data(), slots_.p, cols * sizeof(double));
     return h;
 }
 
@@ -1343,9 +1380,7 @@ std::vector<double> StateVector::probs(const uint32_t *wires, int nw, bool all) 
             launch_probs_gather<decltype(t)>(data(), n_, bits.data(), m, static_cast<double *>(slots_.p), stream_);
         });
         std::vector<double> h(static_cast<size_t>(1) << m);
-        hip_check(hipMemcpyAsync(h.data(), slots_.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_),
-                  "memcpy");
-        hip_check(hipStreamSynchronize(stream_), "sync");
+        read_back(h.data(), slots_.p, h.size() * sizeof(double));
         return h;
     }
     const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(512, std::max<uint64_t>(1, length() / 1024)));
@@ -1479,9 +1514,7 @@ void StateVector::apply_obs_to(const Obs &o, const void *in, void *out) {
             terms.push_back({w.x, w.yz, c.real(), c.imag()});
         }
         small_.ensure(terms.size() * sizeof(PauliTerm) + 256);
-        hip_check(hipMemcpyAsync(small_.p, terms.data(), terms.size() * sizeof(PauliTerm), hipMemcpyHostToDevice,
-                                 stream_),
-                  "memcpy(terms)");
+        upload_small(small_.p, small_.bytes, terms.data(), terms.size() * sizeof(PauliTerm));
         dispatch(dtype_, [&](auto t) {
             launch_pauli_sum<decltype(t)>(in, out, static_cast<const PauliTerm *>(small_.p),
                                           static_cast<uint32_t>(terms.size()), n_, 1, nullptr, 0, stream_);
@@ -1562,9 +1595,7 @@ double StateVector::expval_obs(const Obs &o) {
             terms.push_back({w.x, w.yz, c.real(), c.imag()});
         }
         small_.ensure(terms.size() * sizeof(PauliTerm) + 256);
-        hip_check(hipMemcpyAsync(small_.p, terms.data(), terms.size() * sizeof(PauliTerm), hipMemcpyHostToDevice,
-                                 stream_),
-                  "memcpy(terms)");
+        upload_small(small_.p, small_.bytes, terms.data(), terms.size() * sizeof(PauliTerm));
         const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, length() / 256)));
         partials_.ensure(nb * sizeof(double2));
         dispatch(dtype_, [&](auto t) {
@@ -1573,8 +1604,7 @@ double StateVector::expval_obs(const Obs &o) {
                                           static_cast<double2 *>(partials_.p), nb, stream_);
         });
         std::vector<double2> h(nb);
-        hip_check(hipMemcpyAsync(h.data(), partials_.p, nb * sizeof(double2), hipMemcpyDeviceToHost, stream_), "memcpy");
-        hip_check(hipStreamSynchronize(stream_), "sync");
+        read_back(h.data(), partials_.p, nb * sizeof(double2));
         double s = 0.0, c = 0.0;
         for (const auto &v : h) {
             const double t = s + v.x;
@@ -1713,9 +1743,7 @@ void StateVector::sparse_moments(const CsrView &h, double *mean, double *mean_sq
                                         static_cast<double2 *>(partials_.p), nb, stream_);
     });
     std::vector<double2> hp(2 * static_cast<size_t>(nb));
-    hip_check(hipMemcpyAsync(hp.data(), partials_.p, hp.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_),
-              "memcpy");
-    hip_check(hipStreamSynchronize(stream_), "sync");
+    read_back(hp.data(), partials_.p, hp.size() * sizeof(double2));
     double s0 = 0.0, c0 = 0.0, s1 = 0.0, c1 = 0.0;
     auto add = [](double &sum, double &comp, double x) {
         const double t = sum + x;

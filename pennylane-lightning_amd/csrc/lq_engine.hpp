@@ -328,6 +328,14 @@ class StateVector {
     HostPinned readback_; // dot results, written by a copy kernel (kernel_copies)
     hipEvent_t staged_ = nullptr;
     hipEvent_t copied_ = nullptr; // a copy out of another state finished reading it
+    hipEvent_t bounced_ = nullptr; // the last upload through bounce_ finished reading it
+    HostPinned bounce_;            // small uploads (kernel_copies)
+    // Small host<->device transfers on this state's stream: through a copy
+    // kernel and fine-grained host memory when kernel copies are on (an
+    // upload may write up to dev_cap bytes, rounded up to 16), else
+    // hipMemcpyAsync. read_back returns once the data is in `host`.
+    void upload_small(void *dev, size_t dev_cap, const void *host, size_t bytes);
+    void read_back(void *host, const void *dev, size_t bytes);
     std::vector<std::unique_ptr<DevBuf>> ws_;
     bool prof_ = false;
     struct Pending {
