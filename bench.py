@@ -343,9 +343,15 @@ def serving_sweep(count, qps_batched, qps_plain, bits=128, seed=0, server_kw=Non
         srv.replay(warm_big, np.zeros(len(warm_big)))
         for q in sweep:
             st0 = srv.stats()
+            srv.timing(reset=True)
             res = srv.replay(reqs, S.poisson_arrivals(len(reqs), q, seed=7))
             st1 = srv.stats()
+            tm = srv.timing(reset=True)
             summ = S.summarize(res, len(reqs))
+            if batched:
+                # packed path: longest ready -> launch wait and launch (call to done)
+                summ.update({"packed_ready_wait_max_ms": round(tm["ready_wait_max_ms"], 2),
+                             "packed_launch_max_ms": round(tm["launch_max_ms"], 2)})
             summ.update({"mode": "batched" if batched else "plain", "offered_qps": q,
                          "states_made": st1["states_made"] - st0["states_made"],
                          "states_freed": st1["states_freed"] - st0["states_freed"],
